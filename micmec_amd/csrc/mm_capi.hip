@@ -1,0 +1,350 @@
+// C ABI: device-resident force field objects (see include/micmec_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/micmec_amd.h"
+#include "mm_device.h"
+#include "mm_kernels.h"
+#include "mm_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace mm {
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_error(hipError_t e, const char* what) {
+  return set_error(MM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// 3x3 inverse (row-major) by cofactors; returns the determinant.
+double inverse3(const double* h, double* inv) {
+  const double a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], k = h[7],
+               l = h[8];
+  const double c00 = e * l - f * k, c01 = f * g - d * l, c02 = d * k - e * g;
+  const double det = a * c00 + b * c01 + c * c02;
+  inv[0] = c00 / det;
+  inv[1] = (c * k - b * l) / det;
+  inv[2] = (b * f - c * e) / det;
+  inv[3] = c01 / det;
+  inv[4] = (a * l - c * g) / det;
+  inv[5] = (c * d - a * f) / det;
+  inv[6] = c02 / det;
+  inv[7] = (b * g - a * k) / det;
+  inv[8] = (a * e - b * d) / det;
+  return det;
+}
+
+// Voigt pair (i, j) of index I: 0:(0,0) 1:(1,1) 2:(2,2) 3:(1,2) 4:(0,2) 5:(0,1)
+// (micmec/analysis/tensor.py:93).
+static const int kVoigt[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+
+// Cv[I][J] = mean of C[i][j][k][l] over the orderings (i,j),(j,i) x (k,l),(l,k). With
+// engineering strains e this gives eps:C:eps = e.Cv.e and (C:eps)_ij -> (Cv e)_I for symmetric
+// eps, whatever symmetry C has.
+void voigt_average(const double* C, double* Cv) {
+  for (int I = 0; I < 6; ++I) {
+    for (int J = 0; J < 6; ++J) {
+      const int i = kVoigt[I][0], j = kVoigt[I][1], k = kVoigt[J][0], l = kVoigt[J][1];
+      const double s = C[((i * 3 + j) * 3 + k) * 3 + l] + C[((j * 3 + i) * 3 + k) * 3 + l] +
+                       C[((i * 3 + j) * 3 + l) * 3 + k] + C[((j * 3 + i) * 3 + l) * 3 + k];
+      Cv[6 * I + J] = 0.25 * s;
+    }
+  }
+}
+
+int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_t* nstates,
+                      const double* h0, const double* c0, const double* free_energy,
+                      const double* temp_eff, double boltzmann, std::vector<double>& tab,
+                      std::vector<double>& kt) {
+  tab.assign((size_t)nrows * max_states * kTabStride, 0.0);
+  kt.assign(nrows, 0.0);
+  for (int t = 0; t < nrows; ++t) {
+    if (nstates[t] < 1 || nstates[t] > max_states)
+      return set_error(MM_ERR_ARG, "row %d has %d states (1..%d allowed)", t, nstates[t],
+                       max_states);
+    kt[t] = boltzmann * temp_eff[t];
+    if (nstates[t] > 1 && !(kt[t] > 0.0))
+      return set_error(MM_ERR_ARG, "row %d: multistate cells need a positive effective_temp", t);
+    for (int s = 0; s < nstates[t]; ++s) {
+      const size_t ts = (size_t)t * max_states + s;
+      double* e = &tab[ts * kTabStride];
+      const double det = inverse3(h0 + ts * 9, e);
+      if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+        return set_error(MM_ERR_ARG, "row %d state %d: singular equilibrium cell matrix", t, s);
+      e[9] = model == MM_MODEL_CORNER8 ? 0.125 * det : det;
+      e[10] = free_energy[ts];
+      voigt_average(c0 + ts * 81, e + 12);
+    }
+  }
+  return MM_OK;
+}
+
+}  // namespace mm
+
+using namespace mm;
+
+struct mm_field {
+  int device = 0;
+  int model = 0;
+  int64_t C = 0, N = 0, W = 0;
+  int32_t nrows = 0, max_states = 0;
+  int32_t* cnode = nullptr;
+  int32_t* cwrap = nullptr;
+  int32_t* blk_row = nullptr;
+  int32_t* ncell = nullptr;
+  int32_t* nstates = nullptr;
+  double* tab = nullptr;
+  double* kt = nullptr;
+  double* gcell = nullptr;
+  double* partials = nullptr;
+  int* nanflag = nullptr;
+  std::vector<int64_t> slot_of_cell;  // reference cell index -> work slot
+};
+
+namespace {
+
+template <typename T>
+int dmalloc(T** p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return hip_error(e, "hipMalloc");
+  return MM_OK;
+}
+
+template <typename T>
+int h2d(T* dst, const T* src, size_t count) {
+  if (count == 0) return MM_OK;
+  hipError_t e = hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_error(e, "hipMemcpy");
+  return MM_OK;
+}
+
+void free_field(mm_field* f) {
+  if (!f) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(f->device);
+  (void)hipFree(f->cnode);
+  (void)hipFree(f->cwrap);
+  (void)hipFree(f->blk_row);
+  (void)hipFree(f->ncell);
+  (void)hipFree(f->nstates);
+  (void)hipFree(f->tab);
+  (void)hipFree(f->kt);
+  (void)hipFree(f->gcell);
+  (void)hipFree(f->partials);
+  (void)hipFree(f->nanflag);
+  (void)hipSetDevice(cur);
+  delete f;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    (void)hipGetDevice(&prev);
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* mm_version(void) { return "micmec_amd 0.1 (gfx950, fp64)"; }
+
+const char* mm_last_error(void) { return g_last_error.c_str(); }
+
+int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64_t nnodes,
+                    const int64_t* surrounding_nodes, const int64_t* surrounding_cells,
+                    const uint8_t* wrap_bits, const int32_t* cell_row, int32_t nrows,
+                    int32_t max_states, const int32_t* nstates, const double* h0,
+                    const double* c0, const double* free_energy, const double* temp_eff,
+                    double boltzmann) {
+  g_last_error.clear();
+  if (!out) return set_error(MM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (model != MM_MODEL_CORNER8 && model != MM_MODEL_ORIGINAL)
+    return set_error(MM_ERR_ARG, "unknown model %d", model);
+  if (ncells < 0 || nnodes < 0) return set_error(MM_ERR_ARG, "negative sizes");
+  if (ncells > (1LL << 27)) return set_error(MM_ERR_ARG, "too many cells (%lld)", (long long)ncells);
+  if (nnodes > (1LL << 30)) return set_error(MM_ERR_ARG, "too many nodes");
+  if (nrows < 1 || nrows > 65535 || max_states < 1 || max_states > 64)
+    return set_error(MM_ERR_ARG, "bad parameter table shape %d x %d", nrows, max_states);
+  if (ncells > 0 && (!surrounding_nodes || !wrap_bits || !cell_row))
+    return set_error(MM_ERR_ARG, "missing cell tables");
+  if (nnodes > 0 && !surrounding_cells) return set_error(MM_ERR_ARG, "missing node table");
+  // Validate every index the kernels dereference.
+  for (int64_t c = 0; c < ncells; ++c) {
+    for (int k = 0; k < 8; ++k) {
+      const int64_t n = surrounding_nodes[c * 8 + k];
+      if (n < 0 || n >= nnodes)
+        return set_error(MM_ERR_ARG, "surrounding_nodes[%lld][%d] = %lld out of range",
+                         (long long)c, k, (long long)n);
+    }
+    if (cell_row[c] < 0 || cell_row[c] >= nrows)
+      return set_error(MM_ERR_ARG, "cell %lld has parameter row %d", (long long)c, cell_row[c]);
+    if (wrap_bits[c] > 7) return set_error(MM_ERR_ARG, "wrap bits of cell %lld > 7", (long long)c);
+  }
+  for (int64_t n = 0; n < nnodes; ++n) {
+    for (int k = 0; k < 8; ++k) {
+      const int64_t c = surrounding_cells[n * 8 + k];
+      if (c < -1 || c >= ncells)
+        return set_error(MM_ERR_ARG, "surrounding_cells[%lld][%d] = %lld out of range",
+                         (long long)n, k, (long long)c);
+    }
+  }
+  std::vector<double> tab, kt;
+  int rc = build_param_table(model, nrows, max_states, nstates, h0, c0, free_energy, temp_eff,
+                             boltzmann, tab, kt);
+  if (rc) return rc;
+
+  // Work order: cells grouped by parameter row, reference order inside a row, each group
+  // padded to a whole number of blocks so that every block is type-uniform.
+  std::vector<std::vector<int64_t>> by_row(nrows);
+  for (int64_t c = 0; c < ncells; ++c) by_row[cell_row[c]].push_back(c);
+  int64_t W = 0;
+  for (int t = 0; t < nrows; ++t)
+    W += (int64_t)((by_row[t].size() + kCellBlock - 1) / kCellBlock) * kCellBlock;
+  std::vector<int32_t> cnode((size_t)8 * W, -1), cwrap(W, 0), blk_row(W / kCellBlock, 0);
+  std::vector<int64_t> slot(ncells, -1);
+  int64_t w0 = 0;
+  for (int t = 0; t < nrows; ++t) {
+    const int64_t nb = (int64_t)((by_row[t].size() + kCellBlock - 1) / kCellBlock);
+    for (int64_t b = 0; b < nb; ++b) blk_row[w0 / kCellBlock + b] = t;
+    for (size_t i = 0; i < by_row[t].size(); ++i) {
+      const int64_t c = by_row[t][i];
+      const int64_t w = w0 + (int64_t)i;
+      slot[c] = w;
+      for (int k = 0; k < 8; ++k) cnode[(size_t)k * W + w] = (int32_t)surrounding_nodes[c * 8 + k];
+      cwrap[w] = wrap_bits[c];
+    }
+    w0 += nb * kCellBlock;
+  }
+  std::vector<int32_t> ncell((size_t)8 * nnodes, -1);
+  for (int64_t n = 0; n < nnodes; ++n) {
+    for (int k = 0; k < 8; ++k) {
+      const int64_t c = surrounding_cells[n * 8 + k];
+      if (c >= 0) ncell[(size_t)k * nnodes + n] = (int32_t)(slot[c] * 8 + k);
+    }
+  }
+
+  DeviceGuard guard(device);
+  mm_field* f = new mm_field();
+  f->device = device;
+  f->model = model;
+  f->C = ncells;
+  f->N = nnodes;
+  f->W = W;
+  f->nrows = nrows;
+  f->max_states = max_states;
+  f->slot_of_cell = std::move(slot);
+  const int64_t nb = W / kCellBlock;
+  if ((rc = dmalloc(&f->cnode, (size_t)8 * W)) || (rc = dmalloc(&f->cwrap, W)) ||
+      (rc = dmalloc(&f->blk_row, nb)) || (rc = dmalloc(&f->ncell, (size_t)8 * nnodes)) ||
+      (rc = dmalloc(&f->nstates, nrows)) || (rc = dmalloc(&f->tab, tab.size())) ||
+      (rc = dmalloc(&f->kt, kt.size())) || (rc = dmalloc(&f->gcell, (size_t)24 * W)) ||
+      (rc = dmalloc(&f->partials, (size_t)kPartialStride * (nb > 0 ? nb : 1))) ||
+      (rc = dmalloc(&f->nanflag, 1))) {
+    free_field(f);
+    return rc;
+  }
+  if ((rc = h2d(f->cnode, cnode.data(), cnode.size())) || (rc = h2d(f->cwrap, cwrap.data(), W)) ||
+      (rc = h2d(f->blk_row, blk_row.data(), blk_row.size())) ||
+      (rc = h2d(f->ncell, ncell.data(), ncell.size())) ||
+      (rc = h2d(f->nstates, nstates, (size_t)nrows)) ||
+      (rc = h2d(f->tab, tab.data(), tab.size())) || (rc = h2d(f->kt, kt.data(), kt.size()))) {
+    free_field(f);
+    return rc;
+  }
+  hipError_t e = hipMemset(f->nanflag, 0, sizeof(int));
+  if (e != hipSuccess) {
+    free_field(f);
+    return hip_error(e, "hipMemset");
+  }
+  *out = f;
+  return MM_OK;
+}
+
+int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double* energy,
+                     double* gpos, double* vtens, int accumulate, void* stream) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if (!energy) return set_error(MM_ERR_ARG, "energy output is required");
+  if (f->N > 0 && !pos) return set_error(MM_ERR_ARG, "pos is NULL");
+  if (!rvecs) return set_error(MM_ERR_ARG, "rvecs is NULL");
+  DeviceGuard guard(f->device);
+  ForceArgs a;
+  a.pos = pos;
+  a.rvecs = rvecs;
+  a.cnode = f->cnode;
+  a.cwrap = f->cwrap;
+  a.blk_row = f->blk_row;
+  a.tab = f->tab;
+  a.kt = f->kt;
+  a.nstates = f->nstates;
+  a.max_states = f->max_states;
+  a.W = f->W;
+  a.N = f->N;
+  a.ncell = f->ncell;
+  a.gcell = f->gcell;
+  a.partials = f->partials;
+  a.energy = energy;
+  a.gpos = gpos;
+  a.vtens = vtens;
+  a.accumulate = accumulate;
+  a.model = f->model;
+  a.nanflag = f->nanflag;
+  hipError_t e = launch_force(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_error(e, "force kernels");
+  return MM_OK;
+}
+
+int mm_field_check(mm_field* f, void* stream, int32_t* flags) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  DeviceGuard guard(f->device);
+  int h = 0;
+  hipError_t e = hipMemcpyAsync(&h, f->nanflag, sizeof(int), hipMemcpyDeviceToHost,
+                                (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return hip_error(e, "reading the NaN flag");
+  if (flags) *flags = h;
+  if (h) {
+    e = hipMemsetAsync(f->nanflag, 0, sizeof(int), (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return hip_error(e, "clearing the NaN flag");
+    return set_error(MM_ERR_NAN, "not-a-number in %s%s%s", (h & MM_NAN_ENERGY) ? "energy " : "",
+                     (h & MM_NAN_GPOS) ? "gpos " : "", (h & MM_NAN_VTENS) ? "vtens" : "");
+  }
+  return MM_OK;
+}
+
+int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* nslots) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if (ncells) *ncells = f->C;
+  if (nnodes) *nnodes = f->N;
+  if (nslots) *nslots = f->W;
+  return MM_OK;
+}
+
+void mm_field_destroy(mm_field* f) { free_field(f); }
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Device-side building blocks of the nanocell force field (CDNA4 / gfx950, FP64).
+//
+// Geometry conventions follow micmec/pes/nanocell.py:40-132 and micmec/utils.py:43-52:
+//   corner k of a cell sits at lattice offset OFF[k] = (000,100,010,001,110,101,011,111);
+//   the corner matrix M_a has as rows the three edges through corner a, each oriented along
+//   +x, +y, +z (micmec/pes/nanocell_utils.py:32-75);
+//   B_a = h0^-1 M_a, eps_a = (B_a B_a^T - I)/2 (Green-Lagrange), u_a = eps_a : C0 : eps_a / 2.
+// The gradient is the adjoint of that chain: dE/dB = S B with S the symmetric stress built from
+// the Voigt vector C e, and dE/dM = h0^-T dE/dB; it equals the derivative-table route of
+// nanocell.py:98-132.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mm {
+
+// Doubles per (type, state) entry of the parameter table.
+//   [0..8]   P = h0^-1 (row-major)
+//   [9]      w  = det(h0) / 8 (corner8) or det(h0) (original)
+//   [10]     free energy of the state
+//   [11]     unused
+//   [12..47] Voigt matrix Cv[6][6] (row-major), averaged over index orderings
+constexpr int kTabStride = 48;
+
+// Edge endpoints: corner a, axis d -> lower / upper corner of the edge along d through a.
+__device__ constexpr int kLo[8][3] = {
+    {0, 0, 0}, {0, 1, 1}, {2, 0, 2}, {3, 3, 0}, {2, 1, 4}, {3, 5, 1}, {6, 3, 2}, {6, 5, 4}};
+__device__ constexpr int kHi[8][3] = {
+    {1, 2, 3}, {1, 4, 5}, {4, 2, 6}, {5, 6, 3}, {4, 4, 7}, {5, 7, 5}, {7, 6, 6}, {7, 7, 7}};
+// Corner offsets along x, y, z.
+__device__ constexpr int kOff[8][3] = {
+    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+
+// Strain energy density of one corner (or of the averaged matrix) and, if kGrad, the
+// derivative with respect to the corner matrix M, both for one (type, state) entry `tab`.
+// M is given row-major (rows = edges). Returns u = e.Cv.e / 2; writes G = dU/dM (unscaled).
+template <bool kGrad>
+__device__ __forceinline__ double corner_energy(const double* __restrict__ tab, const double M[9],
+                                                double G[9]) {
+  const double* P = tab;
+  double B[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      B[3 * i + j] = P[3 * i + 0] * M[0 + j] + P[3 * i + 1] * M[3 + j] + P[3 * i + 2] * M[6 + j];
+    }
+  }
+  // Q = B B^T; engineering strain e = (eps00, eps11, eps22, 2eps12, 2eps02, 2eps01)
+  double e[6];
+  e[0] = 0.5 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2] - 1.0);
+  e[1] = 0.5 * (B[3] * B[3] + B[4] * B[4] + B[5] * B[5] - 1.0);
+  e[2] = 0.5 * (B[6] * B[6] + B[7] * B[7] + B[8] * B[8] - 1.0);
+  e[3] = B[3] * B[6] + B[4] * B[7] + B[5] * B[8];
+  e[4] = B[0] * B[6] + B[1] * B[7] + B[2] * B[8];
+  e[5] = B[0] * B[3] + B[1] * B[4] + B[2] * B[5];
+  const double* Cv = tab + 12;
+  double s[6];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    s[I] = Cv[6 * I + 0] * e[0] + Cv[6 * I + 1] * e[1] + Cv[6 * I + 2] * e[2] +
+           Cv[6 * I + 3] * e[3] + Cv[6 * I + 4] * e[4] + Cv[6 * I + 5] * e[5];
+  }
+  const double u = 0.5 * (e[0] * s[0] + e[1] * s[1] + e[2] * s[2] + e[3] * s[3] + e[4] * s[4] +
+                          e[5] * s[5]);
+  if (kGrad) {
+    // S (symmetric): S00=s0 S11=s1 S22=s2 S12=s3 S02=s4 S01=s5 ; T = S B ; G = P^T T
+    const double S[9] = {s[0], s[5], s[4], s[5], s[1], s[3], s[4], s[3], s[2]};
+    double T[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        T[3 * i + j] = S[3 * i + 0] * B[0 + j] + S[3 * i + 1] * B[3 + j] + S[3 * i + 2] * B[6 + j];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        G[3 * i + j] = P[0 + i] * T[0 + j] + P[3 + i] * T[3 + j] + P[6 + i] * T[6 + j];
+      }
+    }
+  }
+  return u;
+}
+
+// Energy of one metastable state of a cell (without the free-energy offset) and, if kGrad,
+// the gradient with respect to the eight vertices, accumulated (scaled by w) into g[8][3].
+// kModel: 0 = corner8 (nanocell.py), 1 = original (nanocell_original.py).
+template <int kModel, bool kGrad>
+__device__ __forceinline__ double state_energy(const double* __restrict__ tab, const double r[8][3],
+                                               double g[8][3]) {
+  const double w = tab[9];
+  if (kModel == 0) {
+    double usum = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      double M[9], G[9];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[3 * d + j] = r[kHi[a][d]][j] - r[kLo[a][d]][j];
+      }
+      usum += corner_energy<kGrad>(tab, M, G);
+      if (kGrad) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double v = w * G[3 * d + j];
+            g[kHi[a][d]][j] += v;
+            g[kLo[a][d]][j] -= v;
+          }
+        }
+      }
+    }
+    return w * usum;
+  } else {
+    // H = (1/8) sum_a M_a: row d is a quarter of the sum of the four edges along d.
+    double H[9], G[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) acc += r[kHi[a][d]][j] - r[kLo[a][d]][j];
+        H[3 * d + j] = 0.125 * acc;
+      }
+    }
+    const double u = corner_energy<kGrad>(tab, H, G);
+    if (kGrad) {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double sgn = kOff[v][d] ? 0.25 : -0.25;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) g[v][j] += sgn * w * G[3 * d + j];
+        }
+      }
+    }
+    return w * u;
+  }
+}
+
+}  // namespace mm

@@ -1,0 +1,259 @@
+// Force-field kernels: per-cell energies and vertex gradients, node gather, reductions.
+//
+// Reference path: ForcePartMechanical._internal_compute (micmec/pes/mmff.py:288-323) and
+// deformation (mmff.py:326-403). Layout in HBM (all FP64 unless noted):
+//   pos      [N][3]            node positions (caller's buffer)
+//   rvecs    [3][3]            domain vectors, rows (device copy)
+//   cnode    [8][W] int32      corner k of work slot w -> node (SoA: coalesced per corner)
+//   cwrap    [W]    int32      wrap bits of the cell in slot w
+//   blk_row  [W/256] int32     parameter-table row shared by all cells of a block
+//   ncell    [8][N] int32      node n -> 8*w + k of the cell in which n is corner k, -1 none
+//   gcell    [8][3][W]         per-(corner, component) vertex gradient of each cell
+//   partials [W/256][16]       per-block energy (0) and virial (1..9)
+// Cells are grouped by type into block-aligned segments, so the type, its number of states and
+// its parameter row are uniform per block and the parameters are read with scalar loads.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mm_device.h"
+#include "mm_kernels.h"
+
+namespace mm {
+
+template <int NV>
+__device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += __shfl_down(v[q], off, 64);
+  }
+}
+
+// Deterministic block reduction: fixed shuffle tree inside each wave, then waves in order.
+template <int NV, int BS>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* smem) {
+  wave_reduce<NV>(v);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) smem[wave * NV + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = smem[q];
+      for (int w = 1; w < BS / 64; ++w) s += smem[w * NV + q];
+      v[q] = s;
+    }
+  }
+}
+
+// Gather the eight vertices of a cell: r_0 = pos[n_0], r_k = r_0 + ((pos[n_k] - pos[n_0])
+// + sum_d shift_kd rvecs[d]) with shift_kd = 1 when the cell is in the last layer along a
+// periodic axis d and corner k lies at +d (mmff.py:347-371 with get_mic 258-286).
+__device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
+                                                const double* __restrict__ rv, const int n[8],
+                                                int wrap, double r[8][3]) {
+  double p0[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) p0[j] = pos[3 * (int64_t)n[0] + j];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) r[0][j] = p0[j];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    const bool sx = kOff[k][0] && (wrap & 1);
+    const bool sy = kOff[k][1] && (wrap & 2);
+    const bool sz = kOff[k][2] && (wrap & 4);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double sh = ((sx ? rv[0 + j] : 0.0) + (sy ? rv[3 + j] : 0.0)) + (sz ? rv[6 + j] : 0.0);
+      const double dvec = (pos[3 * (int64_t)n[k] + j] - p0[j]) + sh;
+      r[k][j] = p0[j] + dvec;
+    }
+  }
+}
+
+// One lane per cell. Energies of all states of the block's cell type, combined with a running
+// log-sum-exp (mmff.py:386-398); the vertex gradient of the combined energy goes to gcell and
+// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials.
+template <int kModel, bool kGrad>
+__global__ void __launch_bounds__(kCellBlock)
+cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
+            const int32_t* __restrict__ cnode, const int32_t* __restrict__ cwrap,
+            const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
+            const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
+            int32_t max_states, int64_t W, double* __restrict__ gcell,
+            double* __restrict__ partials) {
+  __shared__ double red[10 * (kCellBlock / 64)];
+  const int trow = blk_row[blockIdx.x];
+  const int ns = nstates_g[trow];
+  const double kt = kt_g[trow];
+  const double* tab = tab_g + (int64_t)trow * max_states * kTabStride;
+  double rv[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) rv[q] = rvecs[q];
+
+  const int64_t w = (int64_t)blockIdx.x * kCellBlock + threadIdx.x;
+  int n[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) n[k] = cnode[k * W + w];
+  double acc[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  if (n[0] >= 0) {
+    const int wrap = cwrap[w];
+    double r[8][3];
+    gather_vertices(pos, rv, n, wrap, r);
+    double gacc[8][3];
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gacc[v][j] = 0.0;
+    double emin = tab[10] + state_energy<kModel, kGrad>(tab, r, gacc);
+    double zsum = 1.0;
+    for (int s = 1; s < ns; ++s) {
+      const double* ts = tab + s * kTabStride;
+      double gs[8][3];
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gs[v][j] = 0.0;
+      const double es = ts[10] + state_energy<kModel, kGrad>(ts, r, gs);
+      if (es < emin) {
+        const double f = exp(-(emin - es) / kt);
+        zsum = zsum * f + 1.0;
+#pragma unroll
+        for (int v = 0; v < 8; ++v)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) gacc[v][j] = gacc[v][j] * f + gs[v][j];
+        emin = es;
+      } else {
+        const double f = exp(-(es - emin) / kt);
+        zsum += f;
+#pragma unroll
+        for (int v = 0; v < 8; ++v)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) gacc[v][j] += f * gs[v][j];
+      }
+    }
+    double ecell = emin;
+    if (ns > 1) {
+      ecell = emin - kt * log(zsum);
+      const double inv = 1.0 / zsum;
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gacc[v][j] *= inv;
+    }
+    acc[0] = ecell;
+    if (kGrad) {
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gcell[(int64_t)(3 * v + j) * W + w] = gacc[v][j];
+      // virial: vtens[k][l] = sum_v g_v[k] (r_v - r_0)[l]
+#pragma unroll
+      for (int v = 1; v < 8; ++v) {
+        double d[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) d[l] = r[v][l] - r[0][l];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] += gacc[v][k] * d[l];
+      }
+    }
+  }
+  block_reduce<10, kCellBlock>(acc, red);
+  if (threadIdx.x == 0) {
+    double* p = partials + (int64_t)blockIdx.x * kPartialStride;
+#pragma unroll
+    for (int q = 0; q < 10; ++q) p[q] = acc[q];
+  }
+}
+
+// gpos[n] (+)= sum_{k=0..7} gcell[k][:, w(n,k)] in fixed k order (mmff.py:303-318).
+__global__ void __launch_bounds__(256)
+gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell, int64_t N,
+              int64_t W, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  double g[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int32_t e = ncell[k * N + n];
+    if (e >= 0) {
+      const int64_t w = e >> 3;
+      const int kk = e & 7;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
+    }
+  }
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    bad |= (g[j] != g[j]);
+    gpos[3 * n + j] = accumulate ? gpos[3 * n + j] + g[j] : g[j];
+  }
+  if (bad) atomicOr(nanflag, 2);
+}
+
+// Sum block partials in fixed order; write energy / virial (overwrite or accumulate).
+__global__ void __launch_bounds__(256)
+finalize_kernel(const double* __restrict__ partials, int nblocks, double* __restrict__ energy,
+                double* __restrict__ vtens, int accumulate, int* __restrict__ nanflag) {
+  __shared__ double red[10 * 4];
+  double acc[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) acc[q] += partials[(int64_t)b * kPartialStride + q];
+  }
+  block_reduce<10, 256>(acc, red);
+  if (threadIdx.x == 0) {
+    int bad = 0;
+    if (acc[0] != acc[0]) bad |= 1;
+    if (energy) energy[0] = accumulate ? energy[0] + acc[0] : acc[0];
+    if (vtens) {
+      for (int q = 0; q < 9; ++q) {
+        if (acc[1 + q] != acc[1 + q]) bad |= 4;
+        vtens[q] = accumulate ? vtens[q] + acc[1 + q] : acc[1 + q];
+      }
+    }
+    if (bad) atomicOr(nanflag, bad);
+  }
+}
+
+hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
+  const int nb = (int)(a.W / kCellBlock);
+  const bool grad = a.gpos != nullptr || a.vtens != nullptr;
+  if (nb > 0) {
+#define MM_LAUNCH_CELL(MODEL, GRAD)                                                            \
+  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD>), dim3(nb), dim3(kCellBlock), 0, stream, a.pos, \
+                     a.rvecs, a.cnode, a.cwrap, a.blk_row, a.tab, a.kt, a.nstates,             \
+                     a.max_states, a.W, a.gcell, a.partials)
+    if (a.model == 0) {
+      if (grad) MM_LAUNCH_CELL(0, true); else MM_LAUNCH_CELL(0, false);
+    } else {
+      if (grad) MM_LAUNCH_CELL(1, true); else MM_LAUNCH_CELL(1, false);
+    }
+#undef MM_LAUNCH_CELL
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  if (a.gpos != nullptr && a.N > 0 && nb > 0) {
+    const int gb = (int)((a.N + 255) / 256);
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N, a.W,
+                       a.gpos, a.accumulate, a.nanflag);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, stream, a.partials, nb, a.energy,
+                     a.vtens, a.accumulate, a.nanflag);
+  return hipGetLastError();
+}
+
+}  // namespace mm

@@ -1,0 +1,20 @@
+// Host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace mm {
+
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* what);
+double inverse3(const double* h, double* inv);
+void voigt_average(const double* C, double* Cv);
+int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_t* nstates,
+                      const double* h0, const double* c0, const double* free_energy,
+                      const double* temp_eff, double boltzmann, std::vector<double>& tab,
+                      std::vector<double>& kt);
+
+}  // namespace mm

@@ -1,0 +1,37 @@
+// Host-visible launch descriptors of the force-field kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mm {
+
+constexpr int kCellBlock = 256;
+constexpr int kPartialStride = 16;
+
+struct ForceArgs {
+  const double* pos;       // [N][3]
+  const double* rvecs;     // [3][3] device
+  const int32_t* cnode;    // [8][W] corner nodes in work order (-1 = padding slot)
+  const int32_t* cwrap;    // [W] wrap bits of each work slot
+  const int32_t* blk_row;  // [W / kCellBlock] table row of every block
+  const double* tab;       // [rows][max_states][kTabStride]
+  const double* kt;        // [rows] boltzmann * effective temperature
+  const int32_t* nstates;  // [rows]
+  int32_t max_states;
+  int64_t W;               // work slots (multiple of kCellBlock)
+  int64_t N;
+  const int32_t* ncell;    // [8][N] node -> 8*w + k, -1 = none
+  double* gcell;           // [8][3][W]
+  double* partials;        // [W / kCellBlock][kPartialStride]
+  double* energy;          // [1] or null
+  double* gpos;            // [N][3] or null
+  double* vtens;           // [9] or null
+  int accumulate;
+  int model;               // 0 corner8, 1 original
+  int* nanflag;
+};
+
+hipError_t launch_force(const ForceArgs& a, hipStream_t stream);
+
+}  // namespace mm
