@@ -79,6 +79,100 @@ int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* 
 
 void mm_field_destroy(mm_field* f);
 
+/* ------------------------------------------------------------------------------------------
+ * Device MD engine: velocity Verlet with an optional Nose-Hoover chain and MTK barostat,
+ * replacing VerletIntegrator.propagate (micmec/sampling/verlet.py:140-190), NHChain /
+ * NHCThermostat (micmec/sampling/nvt.py:361-532), TBCombination (micmec/sampling/npt.py:52-177)
+ * and MTKBarostat (micmec/sampling/npt.py:513-757). All state lives in device memory owned by
+ * the caller: positions, velocities, gradients [N][3], masses [N], a copy of the positions of
+ * the previous step [N][3], and a block of MM_S_COUNT doubles (indices below) holding every
+ * scalar of the integrator and its hooks.
+ * ------------------------------------------------------------------------------------------ */
+
+#define MM_THERMO_NONE 0
+#define MM_THERMO_NHC 1
+#define MM_BARO_NONE 0
+#define MM_BARO_MTK 1
+
+/* Scalar block layout (doubles). Integer-valued entries are stored as doubles. */
+#define MM_S_TIMESTEP 0
+#define MM_S_NDOF 1
+#define MM_S_BOLTZMANN 2
+#define MM_S_THERMO 3
+#define MM_S_BARO 4
+#define MM_S_CHAIN_LEN 5
+#define MM_S_THERMO_TEMP 6
+#define MM_S_BARO_TEMP 7
+#define MM_S_BARO_PRESS 8
+#define MM_S_MASS_PRESS 9
+#define MM_S_BARO_NDOF 10
+#define MM_S_EXT_PRESS 11
+#define MM_S_NVEC 12
+#define MM_S_CHAIN_MASS 16 /* 8 */
+#define MM_S_CHAIN_POS 24  /* 8 */
+#define MM_S_CHAIN_VEL 32  /* 8 */
+#define MM_S_VEL_PRESS 40  /* 9 */
+#define MM_S_RVECS 49      /* 9 */
+#define MM_S_VOLUME 58
+#define MM_S_EPOT 59
+#define MM_S_VTENS 60 /* 9 */
+#define MM_S_EKIN 69
+#define MM_S_SVV 70 /* 9: sum_n m_n v_n (x) v_n */
+#define MM_S_TIME 79
+#define MM_S_COUNTER 80
+#define MM_S_ECONS_THERMO 81
+#define MM_S_ECONS_BARO 82
+#define MM_S_ECONS_CORR 83
+#define MM_S_ETOT 84
+#define MM_S_ECONS 85
+#define MM_S_TEMP 86
+#define MM_S_PRESS 87
+#define MM_S_PTENS 88 /* 9 */
+#define MM_S_CE 97    /* 5: count, ekin mean, ekin var sum, econs mean, econs var sum */
+#define MM_S_CONS_ERR 102
+#define MM_S_VMAT 103 /* 9: pending velocity transform */
+#define MM_S_VMAT_ON 112
+#define MM_S_XMAT 113 /* 9: pending position transform */
+#define MM_S_CHAINVEL0 122
+#define MM_S_HAS_CHAINVEL0 123
+#define MM_S_RMSD_GPOS 124
+#define MM_S_RMSD_DELTA 125
+#define MM_S_NAN 126
+#define MM_S_EPOT_MECH 127
+#define MM_S_COUNT 128
+
+typedef struct mm_md mm_md;
+
+/* Bind an integrator to a force field and to caller-owned device buffers. thermo/baro select
+ * the hooks (MM_THERMO_*, MM_BARO_*); their parameters are read from the scalar block at every
+ * step. No allocation of the caller's buffers happens here. */
+int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double* gpos,
+                 const double* masses, double* posold, double* scalars, int thermo, int baro);
+
+/* Force evaluation of VerletIntegrator.initialize (verlet.py:123-125) and of
+ * MTKBarostat.init (npt.py:612-614): gpos, epot and, if with_vtens, the virial. */
+int mm_md_init_force(mm_md* md, int with_vtens, void* stream);
+
+/* compute_properties of VerletIntegrator.initialize (verlet.py:136, 171-190): kinetic energy,
+ * temperature, conserved energy, pressure tensor; counter and time unchanged. */
+int mm_md_init_properties(mm_md* md, void* stream);
+
+/* Enqueue nsteps integration steps (verlet.py:140-166 with the hooks' pre/post). With
+ * use_graph != 0 one step is captured once into a HIP graph and replayed. */
+int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream);
+
+/* Apply pending velocity/position transforms so that pos and vel hold the reference state. */
+int mm_md_sync(mm_md* md, void* stream);
+
+/* Copy pos into posold (call before the step whose displacement is reported). */
+int mm_md_snapshot(mm_md* md, void* stream);
+
+/* rmsd_gpos and rmsd_delta = sqrt(mean((pos - posold)^2)) into the scalar block
+ * (verlet.py:172-173). */
+int mm_md_rmsd(mm_md* md, void* stream);
+
+void mm_md_destroy(mm_md* md);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,23 +100,6 @@ int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_
 
 using namespace mm;
 
-struct mm_field {
-  int device = 0;
-  int model = 0;
-  int64_t C = 0, N = 0, W = 0;
-  int32_t nrows = 0, max_states = 0;
-  int32_t* cnode = nullptr;
-  int32_t* cwrap = nullptr;
-  int32_t* blk_row = nullptr;
-  int32_t* ncell = nullptr;
-  int32_t* nstates = nullptr;
-  double* tab = nullptr;
-  double* kt = nullptr;
-  double* gcell = nullptr;
-  double* partials = nullptr;
-  int* nanflag = nullptr;
-  std::vector<int64_t> slot_of_cell;  // reference cell index -> work slot
-};
 
 namespace {
 

@@ -227,28 +227,32 @@ finalize_kernel(const double* __restrict__ partials, int nblocks, double* __rest
   }
 }
 
-hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
+hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   const int nb = (int)(a.W / kCellBlock);
-  const bool grad = a.gpos != nullptr || a.vtens != nullptr;
-  if (nb > 0) {
+  const bool grad = a.gpos != nullptr || a.vtens != nullptr || a.force_grad;
+  if (nb == 0) return hipSuccess;
 #define MM_LAUNCH_CELL(MODEL, GRAD)                                                            \
   hipLaunchKernelGGL((cell_kernel<MODEL, GRAD>), dim3(nb), dim3(kCellBlock), 0, stream, a.pos, \
                      a.rvecs, a.cnode, a.cwrap, a.blk_row, a.tab, a.kt, a.nstates,             \
                      a.max_states, a.W, a.gcell, a.partials)
-    if (a.model == 0) {
-      if (grad) MM_LAUNCH_CELL(0, true); else MM_LAUNCH_CELL(0, false);
-    } else {
-      if (grad) MM_LAUNCH_CELL(1, true); else MM_LAUNCH_CELL(1, false);
-    }
-#undef MM_LAUNCH_CELL
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
+  if (a.model == 0) {
+    if (grad) MM_LAUNCH_CELL(0, true); else MM_LAUNCH_CELL(0, false);
+  } else {
+    if (grad) MM_LAUNCH_CELL(1, true); else MM_LAUNCH_CELL(1, false);
   }
+#undef MM_LAUNCH_CELL
+  return hipGetLastError();
+}
+
+hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
+  const int nb = (int)(a.W / kCellBlock);
+  hipError_t err = launch_cells(a, stream);
+  if (err != hipSuccess) return err;
   if (a.gpos != nullptr && a.N > 0 && nb > 0) {
     const int gb = (int)((a.N + 255) / 256);
     hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N, a.W,
                        a.gpos, a.accumulate, a.nanflag);
-    hipError_t err = hipGetLastError();
+    err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, stream, a.partials, nb, a.energy,

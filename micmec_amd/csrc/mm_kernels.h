@@ -30,8 +30,11 @@ struct ForceArgs {
   int accumulate;
   int model;               // 0 corner8, 1 original
   int* nanflag;
+  int force_grad = 0;       // compute gradients even when gpos and vtens are null
 };
 
 hipError_t launch_force(const ForceArgs& a, hipStream_t stream);
+// Cell kernel only: per-cell gradients into gcell, energy/virial block partials.
+hipError_t launch_cells(const ForceArgs& a, hipStream_t stream);
 
 }  // namespace mm

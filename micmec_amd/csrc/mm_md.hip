@@ -1,0 +1,930 @@
+// Device MD engine: velocity Verlet with Nose-Hoover chain and MTK barostat (FP64).
+//
+// Reference sequence (one step, micmec/sampling/verlet.py:140-166):
+//   pre hooks  -> v += a dt/2 -> x += v dt -> force -> v += a dt/2 -> ekin -> post hooks
+// With a thermostat and a barostat the hooks run as TBCombination (micmec/sampling/npt.py:99-148):
+//   pre  = MTK.baro (npt.py:653-736) then NHC chain (nvt.py:410-451);
+//   post = NHC chain then MTK.baro. Each MTK.baro call evaluates the force once more.
+// Node-wise work runs in wide kernels; every scalar update (chain, barostat velocity, 3x3
+// eigendecompositions, energies, conserved-quantity tracking) runs in a single-block kernel on
+// the scalar block S, between the node kernels, so a whole step can be captured in a HIP graph.
+// Velocity/position transforms produced by a scalar phase (thermostat factor, barostat
+// rotation) are stored in S and applied by the next node pass that touches the array.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../include/micmec_amd.h"
+#include "mm_device.h"
+#include "mm_internal.h"
+#include "mm_kernels.h"
+
+namespace mm {
+
+constexpr int kNodeBlock = 256;
+constexpr int kNodePartial = 16;  // [0] ekin [1..9] sum m v(x)v [10] sum g^2 [11] sum d^2 [12] nan
+
+enum NodeMode { kKickDrift = 0, kScalePos = 1, kScaleVel = 2, kVV = 3, kApplyV = 4, kRmsd = 5 };
+
+enum Phase {
+  kPhInitForce = 0,      // epot (+ vtens) after an initial force evaluation
+  kPhInitProps = 1,      // node sums -> ekin, svv; properties without advancing time
+  kPhNveEnd = 2,         // force + node sums after the second kick; properties
+  kPhNvtPre = 3,         // NHC pre
+  kPhNvtEnd = 4,         // force + node sums; NHC post; properties
+  kPhBaroPre = 5,        // TBC pre, barostat part 1: v_g update, position/domain scaling
+  kPhBaroMid = 6,        // force inside MTK.baro; velocity transform
+  kPhBaroEndPre = 7,     // node sums after the velocity transform; v_g update; NHC pre
+  kPhMainEndNpt = 8,     // force + node sums; NHC post; barostat part 1 of the post call
+  kPhBaroEndPost = 9,    // node sums; v_g update; econs; properties
+  kPhRmsd = 10,          // rmsd_gpos / rmsd_delta
+  kPhApplied = 11,       // pending transforms were applied
+};
+
+// ----------------------------------------------------------------------------- node kernels
+
+template <int NV>
+__device__ __forceinline__ void wave_sum(double (&v)[NV]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += __shfl_down(v[q], off, 64);
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out) {
+  __shared__ double red[NV * (kNodeBlock / 64)];
+  wave_sum<NV>(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave * NV + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = red[q];
+      for (int w = 1; w < kNodeBlock / 64; ++w) s += red[w * NV + q];
+      out[q] = s;
+    }
+  }
+}
+
+// Kinetic-energy and m v (x) v sums of one node, in the reference's operation order:
+// 0.5 * (v**2 * m) (verlet.py:169) and (v.T * m) . v (verlet.py:187-188, npt.py:661-663).
+__device__ __forceinline__ void node_vv(const double v[3], double m, double (&acc)[13]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) acc[0] += 0.5 * ((v[j] * v[j]) * m);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc[1 + 3 * i + j] += (v[i] * m) * v[j];
+}
+
+// row vector times 3x3 matrix
+__device__ __forceinline__ void vecmat(const double v[3], const double* M, double out[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) out[j] = (v[0] * M[j] + v[1] * M[3 + j]) + v[2] * M[6 + j];
+}
+
+__device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, double v[3]) {
+#pragma clang fp contract(off)
+  const int on = (int)S[MM_S_VMAT_ON];
+  if (on == 1) {
+    const double f = S[MM_S_VMAT];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = v[j] * f;
+  } else if (on == 2) {
+    double t[3];
+    vecmat(v, S + MM_S_VMAT, t);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = t[j];
+  }
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(kNodeBlock)
+node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ g,
+            const double* __restrict__ m, const double* __restrict__ xold,
+            const double* __restrict__ S, int64_t N, double* __restrict__ partials) {
+#pragma clang fp contract(off)
+  const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+  double acc[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) acc[q] = 0.0;
+  if (n < N) {
+    if (kMode == kKickDrift) {
+      const double dt = S[MM_S_TIMESTEP];
+      const double mass = m[n];
+      double vv[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
+      apply_vtransform(S, vv);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double a = -g[3 * n + j] / mass;
+        vv[j] = vv[j] + (0.5 * a) * dt;
+        v[3 * n + j] = vv[j];
+        x[3 * n + j] = x[3 * n + j] + dt * vv[j];
+      }
+    } else if (kMode == kScalePos) {
+      double p[3], t[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) p[j] = x[3 * n + j];
+      vecmat(p, S + MM_S_XMAT, t);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) x[3 * n + j] = t[j];
+    } else if (kMode == kScaleVel || kMode == kVV || kMode == kApplyV) {
+      double vv[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
+      if (kMode != kVV) {
+        apply_vtransform(S, vv);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
+      }
+      if (kMode != kApplyV) node_vv(vv, m[n], acc);
+    } else if (kMode == kRmsd) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double gj = g[3 * n + j];
+        const double d = x[3 * n + j] - xold[3 * n + j];
+        acc[10] += gj * gj;
+        acc[11] += d * d;
+      }
+    }
+  }
+  if (kMode == kScaleVel || kMode == kVV || kMode == kRmsd) {
+    block_sum_store<13>(acc, partials + (int64_t)blockIdx.x * kNodePartial);
+  }
+}
+
+// gpos[n] = sum_k gcell[k][:, w(n,k)] (mmff.py:303-318), fused with the second half kick
+// v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities.
+template <bool kKick>
+__global__ void __launch_bounds__(kNodeBlock)
+gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
+                   int64_t N, int64_t W, double* __restrict__ gpos, double* __restrict__ v,
+                   const double* __restrict__ m, const double* __restrict__ S,
+                   double* __restrict__ partials) {
+  const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+  double acc[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) acc[q] = 0.0;
+  if (n < N) {
+    double g[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int32_t e = ncell[k * N + n];
+      if (e >= 0) {
+        const int64_t w = e >> 3;
+        const int kk = e & 7;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      gpos[3 * n + j] = g[j];
+      if (g[j] != g[j]) acc[12] = 1.0;
+    }
+    if (kKick) {
+#pragma clang fp contract(off)
+      const double dt = S[MM_S_TIMESTEP];
+      const double mass = m[n];
+      double vv[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double a = -g[j] / mass;
+        vv[j] = v[3 * n + j] + (0.5 * a) * dt;
+        v[3 * n + j] = vv[j];
+      }
+      node_vv(vv, mass, acc);
+    }
+  }
+  block_sum_store<13>(acc, partials + (int64_t)blockIdx.x * kNodePartial);
+}
+
+// ----------------------------------------------------------------------------- scalar math
+
+struct Sym3 {
+  double a[9];
+};
+
+// Cyclic Jacobi eigendecomposition of a symmetric 3x3 matrix: A = Q diag(w) Q^T.
+__device__ void jacobi3(const double* A_in, double w[3], double Q[9]) {
+  double A[9];
+  for (int i = 0; i < 9; ++i) A[i] = A_in[i];
+  for (int i = 0; i < 9; ++i) Q[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    const double off = fabs(A[1]) + fabs(A[2]) + fabs(A[5]);
+    const double scale = fabs(A[0]) + fabs(A[4]) + fabs(A[8]) + off;
+    if (off == 0.0 || off <= 1e-300 + 1e-18 * scale) break;
+    for (int p = 0; p < 2; ++p) {
+      for (int q = p + 1; q < 3; ++q) {
+        const double apq = A[3 * p + q];
+        if (apq == 0.0) continue;
+        const double app = A[3 * p + p], aqq = A[3 * q + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) /
+                         (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        // A <- J^T A J with J the rotation in the (p, q) plane
+        for (int k = 0; k < 3; ++k) {
+          const double akp = A[3 * k + p], akq = A[3 * k + q];
+          A[3 * k + p] = c * akp - s * akq;
+          A[3 * k + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double apk = A[3 * p + k], aqk = A[3 * q + k];
+          A[3 * p + k] = c * apk - s * aqk;
+          A[3 * q + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double qkp = Q[3 * k + p], qkq = Q[3 * k + q];
+          Q[3 * k + p] = c * qkp - s * qkq;
+          Q[3 * k + q] = s * qkp + c * qkq;
+        }
+      }
+    }
+  }
+  w[0] = A[0];
+  w[1] = A[4];
+  w[2] = A[8];
+}
+
+// R = Q diag(exp((sign * w) * dt / 2)) Q^T for symmetric A (npt.py:686-690, 709-726).
+__device__ void expm_sym_half(const double* A, double sign, double dt, double* R) {
+#pragma clang fp contract(off)
+  double w[3], Q[9];
+  jacobi3(A, w, Q);
+  double e[3];
+  for (int i = 0; i < 3; ++i) e[i] = exp(((sign * w[i]) * dt) / 2.0);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      R[3 * i + j] = ((Q[3 * i + 0] * e[0]) * Q[3 * j + 0] + (Q[3 * i + 1] * e[1]) * Q[3 * j + 1]) +
+                     (Q[3 * i + 2] * e[2]) * Q[3 * j + 2];
+}
+
+__device__ void matmul3(const double* A, const double* B, double* C) {
+#pragma clang fp contract(off)
+  double T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      T[3 * i + j] = (A[3 * i + 0] * B[0 + j] + A[3 * i + 1] * B[3 + j]) + A[3 * i + 2] * B[6 + j];
+  for (int i = 0; i < 9; ++i) C[i] = T[i];
+}
+
+// |det| of the domain vectors, as micmec/pes/domain.c:42-48.
+__device__ double volume3(const double* r) {
+#pragma clang fp contract(off)
+  return fabs(r[0] * (r[4] * r[8] - r[5] * r[7]) + r[1] * (r[5] * r[6] - r[3] * r[8]) +
+              r[2] * (r[3] * r[7] - r[4] * r[6]));
+}
+
+// NHChain.__call__ (nvt.py:410-451): returns the velocity factor; updates ekin in place.
+__device__ double nhc_chain(double* S, double* ekin_io, bool has_g1, double g1) {
+#pragma clang fp contract(off)
+  const int L = (int)S[MM_S_CHAIN_LEN];
+  const double dt = S[MM_S_TIMESTEP];
+  const double temp = S[MM_S_THERMO_TEMP];
+  const double kb = S[MM_S_BOLTZMANN];
+  const double ndof = S[MM_S_NDOF];
+  double* mass = S + MM_S_CHAIN_MASS;
+  double* pos = S + MM_S_CHAIN_POS;
+  double* vel = S + MM_S_CHAIN_VEL;
+  auto bead = [&](int k, double ekin) {
+    double g;
+    if (k == 0) {
+      g = 2.0 * ekin - (ndof * temp) * kb;
+      if (has_g1) g += g1;
+    } else {
+      g = mass[k - 1] * (vel[k - 1] * vel[k - 1]) - temp * kb;
+    }
+    g /= mass[k];
+    if (k == L - 1) {
+      vel[k] += (g * dt) / 4.0;
+    } else {
+      vel[k] *= exp(((-vel[k + 1]) * dt) / 8.0);
+      vel[k] += (g * dt) / 4.0;
+      vel[k] *= exp(((-vel[k + 1]) * dt) / 8.0);
+    }
+  };
+  double ekin = *ekin_io;
+  for (int k = L - 1; k >= 0; --k) bead(k, ekin);
+  for (int k = 0; k < L; ++k) pos[k] += (vel[k] * dt) / 2.0;
+  const double factor = exp(((-vel[0]) * dt) / 2.0);
+  ekin *= factor * factor;
+  for (int k = 0; k < L; ++k) bead(k, ekin);
+  *ekin_io = ekin;
+  return factor;
+}
+
+// NHChain.get_econs_correction (nvt.py:453-458)
+__device__ double nhc_econs(const double* S) {
+#pragma clang fp contract(off)
+  const int L = (int)S[MM_S_CHAIN_LEN];
+  const double kt = S[MM_S_BOLTZMANN] * S[MM_S_THERMO_TEMP];
+  double kin = 0.0, rest = 0.0;
+  for (int k = 0; k < L; ++k)
+    kin += (S[MM_S_CHAIN_VEL + k] * S[MM_S_CHAIN_VEL + k]) * S[MM_S_CHAIN_MASS + k];
+  for (int k = 1; k < L; ++k) rest += S[MM_S_CHAIN_POS + k];
+  return 0.5 * kin + kt * (S[MM_S_NDOF] * S[MM_S_CHAIN_POS] + rest);
+}
+
+// MTKBarostat._compute_ekin_baro (npt.py:754-757, anisotropic)
+__device__ double baro_ekin(const double* S) {
+#pragma clang fp contract(off)
+  double tr = 0.0;
+  for (int i = 0; i < 9; ++i) tr += S[MM_S_VEL_PRESS + i] * S[MM_S_VEL_PRESS + i];
+  return (0.5 * S[MM_S_MASS_PRESS]) * tr;
+}
+
+// MTKBarostat.add_press_cont (npt.py:738-746)
+__device__ double add_press_cont(const double* S) {
+#pragma clang fp contract(off)
+  return 2.0 * baro_ekin(S) - S[MM_S_BARO_NDOF] * (S[MM_S_BARO_TEMP] * S[MM_S_BOLTZMANN]);
+}
+
+// update_baro_vel of MTKBarostat.baro (npt.py:654-681)
+__device__ void update_baro_vel(double* S) {
+#pragma clang fp contract(off)
+  const double dt = S[MM_S_TIMESTEP];
+  double* vp = S + MM_S_VEL_PRESS;
+  const bool has = S[MM_S_HAS_CHAINVEL0] != 0.0;
+  const double cv0 = S[MM_S_CHAINVEL0];
+  if (has) {
+    const double f = exp(((-dt) * cv0) / 8.0);
+    for (int i = 0; i < 9; ++i) vp[i] *= f;
+  }
+  double P[9];
+  for (int i = 0; i < 9; ++i) P[i] = S[MM_S_SVV + i] - S[MM_S_VTENS + i];
+  const double diag = (2.0 * S[MM_S_EKIN]) / S[MM_S_NDOF] - S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      double p = 0.5 * (P[3 * j + i] + P[3 * i + j]);
+      if (i == j) p = p + diag;
+      const double G = p / S[MM_S_MASS_PRESS];
+      vp[3 * i + j] += (G * dt) / 4.0;
+    }
+  }
+  if (has) {
+    const double f = exp(((-dt) * cv0) / 8.0);
+    for (int i = 0; i < 9; ++i) vp[i] *= f;
+  }
+}
+
+// First half of MTKBarostat.baro: v_g update, then positions and domain scaled by
+// exp(v_g dt / 2) (npt.py:683-702). The position transform is left in XMAT.
+__device__ void baro_part1(double* S) {
+  update_baro_vel(S);
+  double R[9];
+  expm_sym_half(S + MM_S_VEL_PRESS, 1.0, S[MM_S_TIMESTEP], R);
+  for (int i = 0; i < 9; ++i) S[MM_S_XMAT + i] = R[i];
+  matmul3(S + MM_S_RVECS, R, S + MM_S_RVECS);
+  S[MM_S_VOLUME] = volume3(S + MM_S_RVECS);
+}
+
+// Velocity transform of MTKBarostat.baro: exp(-(v_g + tr(v_g)/ndof) dt / 2) (npt.py:709-726),
+// composed after any pending velocity factor.
+__device__ void baro_part2(double* S) {
+#pragma clang fp contract(off)
+  double A[9];
+  const double* vp = S + MM_S_VEL_PRESS;
+  const double tr = (vp[0] + vp[4]) + vp[8];
+  for (int i = 0; i < 9; ++i) A[i] = vp[i];
+  const double sh = tr / S[MM_S_NDOF];
+  A[0] += sh;
+  A[4] += sh;
+  A[8] += sh;
+  double R[9];
+  expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], R);
+  const int on = (int)S[MM_S_VMAT_ON];
+  if (on == 1) {
+    const double f = S[MM_S_VMAT];
+    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] = f * R[i];
+  } else if (on == 2) {
+    matmul3(S + MM_S_VMAT, R, S + MM_S_VMAT);
+  } else {
+    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] = R[i];
+  }
+  S[MM_S_VMAT_ON] = 2.0;
+}
+
+__device__ void set_vscale(double* S, double f) {
+  const int on = (int)S[MM_S_VMAT_ON];
+  if (on == 1) {
+    S[MM_S_VMAT] = S[MM_S_VMAT] * f;
+  } else if (on == 2) {
+    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] *= f;
+  } else {
+    S[MM_S_VMAT] = f;
+    S[MM_S_VMAT_ON] = 1.0;
+  }
+}
+
+// Hook econs corrections as summed by compute_properties (verlet.py:176-181, npt.py:135-148).
+__device__ double econs_correction(const double* S) {
+#pragma clang fp contract(off)
+  const bool th = S[MM_S_THERMO] != 0.0, ba = S[MM_S_BARO] != 0.0;
+  if (th && ba) {
+    const double kt = S[MM_S_BOLTZMANN] * S[MM_S_THERMO_TEMP];
+    return (S[MM_S_ECONS_THERMO] + S[MM_S_ECONS_BARO]) +
+           S[MM_S_BARO_NDOF] * kt * S[MM_S_CHAIN_POS];
+  }
+  if (th) return S[MM_S_ECONS_THERMO];
+  if (ba) return S[MM_S_ECONS_BARO];
+  return 0.0;
+}
+
+// VerletIntegrator.compute_properties (verlet.py:171-190) minus the rmsd terms.
+__device__ void properties(double* S, bool advance) {
+#pragma clang fp contract(off)
+  if (advance) {
+    S[MM_S_TIME] += S[MM_S_TIMESTEP];
+    S[MM_S_COUNTER] += 1.0;
+  }
+  const double ekin = S[MM_S_EKIN];
+  S[MM_S_TEMP] = (ekin / S[MM_S_NDOF]) * (2.0 / S[MM_S_BOLTZMANN]);
+  S[MM_S_ETOT] = ekin + S[MM_S_EPOT];
+  S[MM_S_ECONS_CORR] = econs_correction(S);
+  const double econs = S[MM_S_ETOT] + S[MM_S_ECONS_CORR];
+  S[MM_S_ECONS] = econs;
+  double* ce = S + MM_S_CE;  // ConsErrTracker (verlet.py:275-307)
+  if (ce[0] == 0.0) {
+    ce[1] = ekin;
+    ce[3] = econs;
+  } else {
+    const double t1 = ekin - ce[1];
+    ce[1] += t1 / (ce[0] + 1.0);
+    ce[2] += t1 * (ekin - ce[1]);
+    const double t2 = econs - ce[3];
+    ce[3] += t2 / (ce[0] + 1.0);
+    ce[4] += t2 * (econs - ce[3]);
+  }
+  ce[0] += 1.0;
+  S[MM_S_CONS_ERR] = ce[0] > 1.0 ? sqrt(ce[4] / ce[2]) : 0.0;
+  if (S[MM_S_NVEC] > 0.0) {
+    const double V = S[MM_S_VOLUME];
+    for (int i = 0; i < 9; ++i) S[MM_S_PTENS + i] = (S[MM_S_SVV + i] - S[MM_S_VTENS + i]) / V;
+    S[MM_S_PRESS] = ((S[MM_S_PTENS + 0] + S[MM_S_PTENS + 4]) + S[MM_S_PTENS + 8]) / 3.0;
+  }
+}
+
+// Energy and virial from the force partials, plus the external-pressure part (P V, P V I).
+__device__ void take_force(double* S, const double (&f)[10], bool with_vtens) {
+#pragma clang fp contract(off)
+  S[MM_S_EPOT_MECH] = f[0];
+  const double P = S[MM_S_EXT_PRESS];
+  const double pv = P * S[MM_S_VOLUME];
+  S[MM_S_EPOT] = P != 0.0 ? f[0] + S[MM_S_VOLUME] * P : f[0];
+  if (with_vtens) {
+    for (int q = 0; q < 9; ++q) S[MM_S_VTENS + q] = f[1 + q];
+    if (P != 0.0) {
+      S[MM_S_VTENS + 0] += pv;
+      S[MM_S_VTENS + 4] += pv;
+      S[MM_S_VTENS + 8] += pv;
+    }
+  }
+  if (f[0] != f[0]) S[MM_S_NAN] = 1.0;
+}
+
+__device__ void take_nodes(double* S, const double (&nd)[13]) {
+  S[MM_S_EKIN] = nd[0];
+  for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] = nd[1 + q];
+  if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
+}
+
+template <int NV>
+__device__ void reduce_partials(const double* __restrict__ part, int nb, int stride,
+                                double (&out)[NV]) {
+  __shared__ double red[NV * 4];
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += part[(int64_t)b * stride + q];
+  }
+  wave_sum<NV>(acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave * NV + q] = acc[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[q] = ((red[q] + red[NV + q]) + red[2 * NV + q]) + red[3 * NV + q];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256)
+scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpart, int nbf,
+              const double* __restrict__ npart, int nbn, int with_vtens, int64_t N) {
+  double f[10], nd[13];
+  const bool use_f = phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
+                     phase == kPhBaroMid || phase == kPhMainEndNpt;
+  const bool use_n = phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
+                     phase == kPhBaroEndPre || phase == kPhMainEndNpt ||
+                     phase == kPhBaroEndPost || phase == kPhRmsd;
+  if (use_f) reduce_partials<10>(fpart, nbf, kPartialStride, f);
+  if (use_n) reduce_partials<13>(npart, nbn, kNodePartial, nd);
+  if (threadIdx.x != 0) return;
+  const bool thermo = S[MM_S_THERMO] != 0.0;
+  switch (phase) {
+    case kPhInitForce:
+      take_force(S, f, with_vtens != 0);
+      break;
+    case kPhInitProps:
+      take_nodes(S, nd);
+      properties(S, false);
+      break;
+    case kPhNveEnd:
+      S[MM_S_VMAT_ON] = 0.0;
+      take_force(S, f, true);
+      take_nodes(S, nd);
+      properties(S, true);
+      break;
+    case kPhNvtPre: {
+      double ekin = S[MM_S_EKIN];
+      const double fac = nhc_chain(S, &ekin, false, 0.0);
+      S[MM_S_EKIN] = ekin;
+      set_vscale(S, fac);
+      break;
+    }
+    case kPhNvtEnd: {
+      S[MM_S_VMAT_ON] = 0.0;
+      take_force(S, f, true);
+      take_nodes(S, nd);
+      double ekin = S[MM_S_EKIN];
+      const double fac = nhc_chain(S, &ekin, false, 0.0);
+      set_vscale(S, fac);
+      for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
+      S[MM_S_EKIN] = ekin;
+      S[MM_S_ECONS_THERMO] = nhc_econs(S);
+      properties(S, true);
+      break;
+    }
+    case kPhBaroPre:
+      // TBC.pre: chainvel0 = chain.vel[0] before the barostat (npt.py:101-107)
+      S[MM_S_HAS_CHAINVEL0] = thermo ? 1.0 : 0.0;
+      S[MM_S_CHAINVEL0] = thermo ? S[MM_S_CHAIN_VEL] : 0.0;
+      // a pending thermostat factor scales m v (x) v by its square
+      if ((int)S[MM_S_VMAT_ON] == 1) {
+        const double fac = S[MM_S_VMAT];
+        for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
+      }
+      baro_part1(S);
+      break;
+    case kPhBaroMid:
+      take_force(S, f, true);
+      baro_part2(S);
+      break;
+    case kPhBaroEndPre: {
+      S[MM_S_VMAT_ON] = 0.0;
+      take_nodes(S, nd);
+      update_baro_vel(S);
+      if (thermo) {
+        const double g1 = add_press_cont(S);
+        double ekin = S[MM_S_EKIN];
+        const double fac = nhc_chain(S, &ekin, true, g1);
+        S[MM_S_EKIN] = ekin;
+        set_vscale(S, fac);
+      }
+      break;
+    }
+    case kPhMainEndNpt: {
+      S[MM_S_VMAT_ON] = 0.0;
+      take_force(S, f, true);
+      take_nodes(S, nd);
+      if (thermo) {
+        const double g1 = add_press_cont(S);
+        double ekin = S[MM_S_EKIN];
+        const double fac = nhc_chain(S, &ekin, true, g1);
+        S[MM_S_EKIN] = ekin;
+        set_vscale(S, fac);
+        for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
+        S[MM_S_ECONS_THERMO] = nhc_econs(S);
+      }
+      // TBC.post: chainvel0 after the thermostat (npt.py:128-133)
+      S[MM_S_HAS_CHAINVEL0] = thermo ? 1.0 : 0.0;
+      S[MM_S_CHAINVEL0] = thermo ? S[MM_S_CHAIN_VEL] : 0.0;
+      baro_part1(S);
+      break;
+    }
+    case kPhBaroEndPost:
+      S[MM_S_VMAT_ON] = 0.0;
+      take_nodes(S, nd);
+      update_baro_vel(S);
+      S[MM_S_ECONS_BARO] = baro_ekin(S) + S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
+      properties(S, true);
+      break;
+    case kPhRmsd: {
+      const double denom = 3.0 * (double)N;
+      S[MM_S_RMSD_GPOS] = sqrt(nd[10] / denom);
+      S[MM_S_RMSD_DELTA] = sqrt(nd[11] / denom);
+      break;
+    }
+    case kPhApplied:
+      S[MM_S_VMAT_ON] = 0.0;
+      break;
+  }
+}
+
+}  // namespace mm
+
+using namespace mm;
+
+struct mm_md {
+  mm_field* field = nullptr;
+  double* pos = nullptr;
+  double* vel = nullptr;
+  double* gpos = nullptr;
+  const double* masses = nullptr;
+  double* posold = nullptr;
+  double* S = nullptr;
+  double* npart = nullptr;
+  int thermo = 0, baro = 0;
+  int64_t N = 0;
+  int nbn = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+struct Dev {
+  int prev = -1;
+  explicit Dev(int d) {
+    (void)hipGetDevice(&prev);
+    if (prev != d) (void)hipSetDevice(d);
+  }
+  ~Dev() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+ForceArgs force_args(mm_md* md) {
+  mm_field* f = md->field;
+  ForceArgs a;
+  a.pos = md->pos;
+  a.rvecs = md->S + MM_S_RVECS;
+  a.cnode = f->cnode;
+  a.cwrap = f->cwrap;
+  a.blk_row = f->blk_row;
+  a.tab = f->tab;
+  a.kt = f->kt;
+  a.nstates = f->nstates;
+  a.max_states = f->max_states;
+  a.W = f->W;
+  a.N = f->N;
+  a.ncell = f->ncell;
+  a.gcell = f->gcell;
+  a.partials = f->partials;
+  a.energy = nullptr;
+  a.gpos = nullptr;
+  a.vtens = nullptr;
+  a.accumulate = 0;
+  a.model = f->model;
+  a.nanflag = f->nanflag;
+  a.force_grad = 1;
+  return a;
+}
+
+hipError_t node(mm_md* md, int mode, hipStream_t s) {
+  const dim3 grid(md->nbn), block(kNodeBlock);
+#define MM_NODE(M)                                                                          \
+  hipLaunchKernelGGL((node_kernel<M>), grid, block, 0, s, md->pos, md->vel, md->gpos,        \
+                     md->masses, md->posold, md->S, md->N, md->npart)
+  switch (mode) {
+    case kKickDrift: MM_NODE(kKickDrift); break;
+    case kScalePos: MM_NODE(kScalePos); break;
+    case kScaleVel: MM_NODE(kScaleVel); break;
+    case kVV: MM_NODE(kVV); break;
+    case kApplyV: MM_NODE(kApplyV); break;
+    case kRmsd: MM_NODE(kRmsd); break;
+  }
+#undef MM_NODE
+  return hipGetLastError();
+}
+
+hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
+  const int nbf = (int)(md->field->W / kCellBlock);
+  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(256), 0, s, phase, md->S,
+                     md->field->partials, nbf, md->npart, md->nbn, with_vtens, md->N);
+  return hipGetLastError();
+}
+
+// force evaluation: cells, then gather (+ second half kick when kick)
+hipError_t force(mm_md* md, bool kick, hipStream_t s) {
+  ForceArgs a = force_args(md);
+  hipError_t e = launch_cells(a, s);
+  if (e != hipSuccess) return e;
+  if (md->N > 0) {
+    if (kick) {
+      hipLaunchKernelGGL((gather_kick_kernel<true>), dim3(md->nbn), dim3(kNodeBlock), 0, s,
+                         a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,
+                         md->npart);
+    } else {
+      hipLaunchKernelGGL((gather_kick_kernel<false>), dim3(md->nbn), dim3(kNodeBlock), 0, s,
+                         a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,
+                         md->npart);
+    }
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+#define MM_TRY(x)                         \
+  do {                                    \
+    hipError_t _e = (x);                  \
+    if (_e != hipSuccess) return _e;      \
+  } while (0)
+
+// The barostat call MTKBarostat.baro after its first scalar phase.
+hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s) {
+  MM_TRY(node(md, kScalePos, s));
+  MM_TRY(force(md, false, s));
+  MM_TRY(scalar(md, kPhBaroMid, 1, s));
+  MM_TRY(node(md, kScaleVel, s));
+  MM_TRY(scalar(md, end_phase, 1, s));
+  return hipSuccess;
+}
+
+hipError_t enqueue_step(mm_md* md, hipStream_t s) {
+  if (md->baro) {
+    MM_TRY(scalar(md, kPhBaroPre, 1, s));
+    MM_TRY(baro_call(md, kPhBaroEndPre, s));
+    MM_TRY(node(md, kKickDrift, s));
+    MM_TRY(force(md, true, s));
+    MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
+    MM_TRY(baro_call(md, kPhBaroEndPost, s));
+  } else if (md->thermo) {
+    MM_TRY(scalar(md, kPhNvtPre, 1, s));
+    MM_TRY(node(md, kKickDrift, s));
+    MM_TRY(force(md, true, s));
+    MM_TRY(scalar(md, kPhNvtEnd, 1, s));
+  } else {
+    MM_TRY(node(md, kKickDrift, s));
+    MM_TRY(force(md, true, s));
+    MM_TRY(scalar(md, kPhNveEnd, 1, s));
+  }
+  return hipSuccess;
+}
+
+int begin(mm_md* md, void* stream) {
+  hipError_t e = hipEventRecord(md->ev_in, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(md->stream, md->ev_in, 0);
+  return e == hipSuccess ? MM_OK : hip_error(e, "stream ordering");
+}
+
+int end(mm_md* md, void* stream) {
+  hipError_t e = hipEventRecord(md->ev_out, md->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent((hipStream_t)stream, md->ev_out, 0);
+  return e == hipSuccess ? MM_OK : hip_error(e, "stream ordering");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double* gpos,
+                 const double* masses, double* posold, double* scalars, int thermo, int baro) {
+  if (!out || !field || !scalars) return set_error(MM_ERR_ARG, "missing argument");
+  if (field->N > 0 && (!pos || !vel || !gpos || !masses || !posold))
+    return set_error(MM_ERR_ARG, "missing node buffer");
+  if (thermo != MM_THERMO_NONE && thermo != MM_THERMO_NHC)
+    return set_error(MM_ERR_ARG, "unknown thermostat %d", thermo);
+  if (baro != MM_BARO_NONE && baro != MM_BARO_MTK)
+    return set_error(MM_ERR_ARG, "unknown barostat %d", baro);
+  Dev dev(field->device);
+  mm_md* md = new mm_md();
+  md->field = field;
+  md->pos = pos;
+  md->vel = vel;
+  md->gpos = gpos;
+  md->masses = masses;
+  md->posold = posold;
+  md->S = scalars;
+  md->thermo = thermo;
+  md->baro = baro;
+  md->N = field->N;
+  md->nbn = (int)((field->N + kNodeBlock - 1) / kNodeBlock);
+  if (md->nbn == 0) md->nbn = 1;
+  hipError_t e = hipMalloc((void**)&md->npart, sizeof(double) * kNodePartial * md->nbn);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&md->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&md->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&md->ev_out, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    mm_md_destroy(md);
+    return hip_error(e, "mm_md_create");
+  }
+  *out = md;
+  return MM_OK;
+}
+
+int mm_md_init_force(mm_md* md, int with_vtens, void* stream) {
+  if (!md) return set_error(MM_ERR_ARG, "md is NULL");
+  Dev dev(md->field->device);
+  int rc = begin(md, stream);
+  if (rc) return rc;
+  hipError_t e = force(md, false, md->stream);
+  if (e == hipSuccess) e = scalar(md, kPhInitForce, with_vtens, md->stream);
+  if (e != hipSuccess) return hip_error(e, "mm_md_init_force");
+  return end(md, stream);
+}
+
+int mm_md_init_properties(mm_md* md, void* stream) {
+  if (!md) return set_error(MM_ERR_ARG, "md is NULL");
+  Dev dev(md->field->device);
+  int rc = begin(md, stream);
+  if (rc) return rc;
+  hipError_t e = node(md, kVV, md->stream);
+  if (e == hipSuccess) e = scalar(md, kPhInitProps, 1, md->stream);
+  if (e != hipSuccess) return hip_error(e, "mm_md_init_properties");
+  return end(md, stream);
+}
+
+int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream) {
+  if (!md) return set_error(MM_ERR_ARG, "md is NULL");
+  if (nsteps < 0) return set_error(MM_ERR_ARG, "negative step count");
+  if (nsteps == 0) return MM_OK;
+  Dev dev(md->field->device);
+  int rc = begin(md, stream);
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
+  if (use_graph) {
+    if (!md->exec) {
+      e = hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        hipError_t e2 = enqueue_step(md, md->stream);
+        e = hipStreamEndCapture(md->stream, &md->graph);
+        if (e2 != hipSuccess) e = e2;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) return hip_error(e, "capturing the MD step");
+    }
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = hipGraphLaunch(md->exec, md->stream);
+  } else {
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = enqueue_step(md, md->stream);
+  }
+  if (e != hipSuccess) return hip_error(e, "mm_md_step");
+  return end(md, stream);
+}
+
+int mm_md_sync(mm_md* md, void* stream) {
+  if (!md) return set_error(MM_ERR_ARG, "md is NULL");
+  Dev dev(md->field->device);
+  int rc = begin(md, stream);
+  if (rc) return rc;
+  hipError_t e = node(md, kApplyV, md->stream);
+  if (e == hipSuccess) e = scalar(md, kPhApplied, 0, md->stream);
+  if (e != hipSuccess) return hip_error(e, "mm_md_sync");
+  return end(md, stream);
+}
+
+int mm_md_snapshot(mm_md* md, void* stream) {
+  if (!md) return set_error(MM_ERR_ARG, "md is NULL");
+  Dev dev(md->field->device);
+  int rc = begin(md, stream);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(md->posold, md->pos, sizeof(double) * 3 * md->N,
+                                hipMemcpyDeviceToDevice, md->stream);
+  if (e != hipSuccess) return hip_error(e, "mm_md_snapshot");
+  return end(md, stream);
+}
+
+int mm_md_rmsd(mm_md* md, void* stream) {
+  if (!md) return set_error(MM_ERR_ARG, "md is NULL");
+  Dev dev(md->field->device);
+  int rc = begin(md, stream);
+  if (rc) return rc;
+  hipError_t e = node(md, kRmsd, md->stream);
+  if (e == hipSuccess) e = scalar(md, kPhRmsd, 0, md->stream);
+  if (e != hipSuccess) return hip_error(e, "mm_md_rmsd");
+  return end(md, stream);
+}
+
+void mm_md_destroy(mm_md* md) {
+  if (!md) return;
+  if (md->exec) (void)hipGraphExecDestroy(md->exec);
+  if (md->graph) (void)hipGraphDestroy(md->graph);
+  if (md->stream) {
+    (void)hipStreamSynchronize(md->stream);
+    (void)hipStreamDestroy(md->stream);
+  }
+  if (md->ev_in) (void)hipEventDestroy(md->ev_in);
+  if (md->ev_out) (void)hipEventDestroy(md->ev_out);
+  if (md->npart) (void)hipFree(md->npart);
+  delete md;
+}
+
+}  // extern "C"

@@ -1,0 +1,14 @@
+"""Integrators, thermostats, barostats and optimisers (micmec/sampling)."""
+
+from .iterative import (Iterative, StateItem, AttributeStateItem, PosStateItem,
+                        TemperatureStateItem, VolumeStateItem, DomainStateItem,
+                        ConsErrStateItem, EPotContribStateItem, Hook)
+from .verlet import VerletIntegrator, VerletHook, VerletScreenLog, ConsErrTracker
+from .nvt import NHChain, NHCThermostat, NHCAttributeStateItem
+from .npt import TBCombination, MTKBarostat, MTKAttributeStateItem
+
+__all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
+           "TemperatureStateItem", "VolumeStateItem", "DomainStateItem", "ConsErrStateItem",
+           "EPotContribStateItem", "Hook", "VerletIntegrator", "VerletHook", "VerletScreenLog",
+           "ConsErrTracker", "NHChain", "NHCThermostat", "NHCAttributeStateItem",
+           "TBCombination", "MTKBarostat", "MTKAttributeStateItem"]

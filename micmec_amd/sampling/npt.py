@@ -1,0 +1,137 @@
+"""MTK barostat and the thermostat-barostat combination (micmec/sampling/npt.py:52-177, 513-787).
+
+``MTKBarostat`` (anisotropic, full 3x3 cell fluctuation, no volume constraint, no barostat
+thermostat) and ``TBCombination`` keep the reference constructors. Host-side ``init`` mirrors the
+reference (domain symmetrisation at construction, momentum cleaning, barostat mass, random
+symmetric barostat velocity); the per-step barostat updates, including the extra force
+evaluation of each ``baro`` call, run inside the device integrator.
+"""
+
+import numpy as np
+
+from ..units import boltzmann, femtosecond
+from .iterative import StateItem
+from .nvt import NHCThermostat
+from .utils import (clean_momenta, domain_symmetrize, get_ndof_baro, get_ndof_internal_md,
+                    get_random_vel_press)
+from .verlet import VerletHook
+
+__all__ = ["TBCombination", "MTKBarostat", "MTKAttributeStateItem"]
+
+
+class TBCombination(VerletHook):
+    """Thermostat and barostat called in the order of npt.py:99-148."""
+    name = "TBCombination"
+    _device_hook = True
+
+    def __init__(self, thermostat, barostat, start=0):
+        self.thermostat = thermostat
+        self.barostat = barostat
+        self.start = start
+        if not self.verify():
+            self.barostat = thermostat
+            self.thermostat = barostat
+            if not self.verify():
+                raise TypeError("The Thermostat or Barostat instance is not supported (yet).")
+        self.step_thermo = self.thermostat.step
+        self.step_baro = self.barostat.step
+        VerletHook.__init__(self, start, min(self.step_thermo, self.step_baro))
+
+    def init(self, iterative):
+        self.thermostat.init(iterative)
+        self.barostat.init(iterative)
+        self.chainvel0 = None
+        self.G1_add = None
+
+    def verify(self):
+        return isinstance(self.thermostat, NHCThermostat) and \
+            isinstance(self.barostat, MTKBarostat)
+
+    def pre(self, iterative):
+        raise NotImplementedError("TBC updates run inside the device integrator.")
+
+    post = pre
+
+
+class MTKBarostat(VerletHook):
+    """Martyna-Tobias-Klein barostat with anisotropic cell fluctuations (npt.py:513-757)."""
+    name = "MTTK"
+    kind = "deterministic"
+    method = "barostat"
+    _device_hook = True
+
+    def __init__(self, mmf, temp, press, start=0, step=1, timecon=1000 * femtosecond,
+                 anisotropic=True, vol_constraint=False, baro_thermo=None, vel_press0=None,
+                 restart=False):
+        if not anisotropic or vol_constraint or baro_thermo is not None:
+            raise NotImplementedError("The device MTK barostat supports anisotropic cell "
+                                      "fluctuations without volume constraint or barostat "
+                                      "thermostat.")
+        self.temp = temp
+        self.press = press
+        self.timecon_press = timecon
+        self.anisotropic = anisotropic
+        self.vol_constraint = vol_constraint
+        self.baro_thermo = baro_thermo
+        self.dim = mmf.system.domain.nvec
+        if self.dim != 3:
+            raise ValueError("The MTK barostat needs a 3D periodic domain.")
+        self.restart = restart
+        self.baro_ndof = get_ndof_baro(self.dim, self.anisotropic, self.vol_constraint)
+        if self.anisotropic and not self.restart:
+            domain_symmetrize(mmf)
+        self.domain = np.asarray(mmf.system.domain.rvecs).copy()
+        self.vel_press = None if vel_press0 is None else np.array(vel_press0, dtype=float)
+        self.econs_correction = 0.0
+        VerletHook.__init__(self, start, step)
+
+    def init(self, iterative):
+        self.timestep_press = iterative.timestep
+        if not self.restart:
+            clean_momenta(iterative.pos, iterative.vel, iterative.masses,
+                          iterative.mmf.system.domain)
+        if iterative.ndof is None:
+            iterative.ndof = get_ndof_internal_md(iterative.mmf.system.nnodes,
+                                                  iterative.mmf.system.domain.nvec)
+        angfreq = 2 * np.pi / self.timecon_press
+        self.mass_press = (iterative.ndof + self.dim ** 2) * boltzmann * self.temp / angfreq ** 2
+        if self.vel_press is None:
+            self.vel_press = get_random_vel_press(self.mass_press, self.temp)
+        # gpos and vtens after the symmetrisation (npt.py:611-614)
+        iterative._init_force(with_vtens=True)
+
+    def pre(self, iterative, chainvel0=None):
+        raise NotImplementedError("MTK updates run inside the device integrator.")
+
+    post = pre
+
+    def add_press_cont(self):
+        return 2 * self._compute_ekin_baro() - self.baro_ndof * (self.temp * boltzmann)
+
+    def _compute_ekin_baro(self):
+        return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
+
+
+class MTKAttributeStateItem(StateItem):
+    def __init__(self, attr):
+        StateItem.__init__(self, "baro_" + attr)
+        self.attr = attr
+
+    def get_value(self, iterative):
+        baro = None
+        for hook in iterative.hooks:
+            if isinstance(hook, MTKBarostat):
+                baro = hook
+                break
+            if isinstance(hook, TBCombination):
+                if isinstance(hook.barostat, MTKBarostat):
+                    baro = hook.barostat
+                break
+        if baro is None:
+            raise TypeError("Iterative does not contain an MTKBarostat hook.")
+        if self.key.startswith("baro_chain_"):
+            return 0
+        return getattr(baro, self.attr)
+
+    def copy(self):
+        return self.__class__(self.attr)

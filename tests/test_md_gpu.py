@@ -1,0 +1,158 @@
+"""GPU MD parity: the device integrator against the CPU oracle integrator.
+
+Same seeded system, same numpy random stream (velocities, chain and barostat velocities are
+drawn in the reference's order by both sides), 100 steps of dt = 1 fs. Bar: every node position
+within 1e-8 Angstrom, velocities, energies and hook state within tight relative tolerances.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from micmec_amd import units
+from micmec_amd.pes import ForcePartMechanical, ForcePartPressure, MicMecForceField
+from micmec_amd.sampling import (MTKBarostat, NHCThermostat, VerletIntegrator, Hook)
+from micmec_amd.topology import Grid
+from micmec_amd.workloads import make_system
+from oracle import mmff as omm
+from oracle.integrators import OracleMD
+
+pytestmark = pytest.mark.gpu
+
+FS = units.femtosecond
+ANG = units.angstrom
+
+
+def oracle_force(system, model):
+    wb = Grid(np.rint(system.grid).astype(int), pbc=len(system.domain.rvecs) > 0).wrap_bits()
+    sn, sc, types, params = (system.surrounding_nodes, system.surrounding_cells, system.types,
+                             system.params)
+
+    def force(pos, rvecs):
+        return omm.compute_vectorised(pos, rvecs, types, sn, sc, wb, params, model)
+    return force
+
+
+def build(shape, p_b, p_hole, seed, ensemble, model, ext_press=0.0, press=1.0 * units.bar):
+    s = make_system(shape, p_b=p_b, p_hole=p_hole, seed=seed)
+    pos0 = s.pos.copy()
+    rvecs0 = np.array(s.domain.rvecs).copy()
+    parts = [ForcePartMechanical(s, model=model)]
+    if ext_press:
+        parts.append(ForcePartPressure(s, ext_press))
+    mmf = MicMecForceField(s, parts)
+    np.random.seed(seed)
+    hooks = []
+    thermo = baro = None
+    if "nhc" in ensemble:
+        hooks.append(NHCThermostat(temp=300.0, timecon=100 * FS))
+        thermo = dict(temp=300.0, timecon=100 * FS, chainlength=3)
+    if "mtk" in ensemble:
+        hooks.append(MTKBarostat(mmf, temp=300.0, press=press, timecon=1000 * FS))
+        baro = dict(temp=300.0, press=press, timecon=1000 * FS)
+    verlet = VerletIntegrator(mmf, timestep=1.0 * FS, hooks=hooks, temp0=300.0)
+    np.random.seed(seed)
+    omd = OracleMD(oracle_force(s, model), pos0, rvecs0, s.masses, 1.0 * FS, temp0=300.0,
+                   thermo=thermo, baro=baro, ext_press=ext_press)
+    return s, verlet, omd
+
+
+def compare(verlet, omd, tag):
+    dpos = np.abs(verlet.pos - omd.pos).max() / ANG
+    assert dpos < 1e-8, (tag, "pos", dpos)
+    vscale = np.abs(omd.vel).max()
+    assert np.abs(verlet.vel - omd.vel).max() < 1e-8 * vscale, (tag, "vel")
+    for name in ("epot", "ekin", "econs", "etot", "temp"):
+        a, b = getattr(verlet, name), getattr(omd, name)
+        assert abs(a - b) <= 1e-9 * abs(b) + 1e-15, (tag, name, a, b)
+    assert abs(verlet.cons_err - omd.cons_err) <= 1e-6 * max(abs(omd.cons_err), 1e-12), tag
+    if omd.nvec > 0:
+        assert np.abs(verlet.ptens - omd.ptens).max() <= 1e-8 * np.abs(omd.ptens).max(), tag
+    assert abs(verlet.time - omd.time) < 1e-9 * omd.time + 1e-12
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "nvt-nhc", "npt-nhc-mtk", "nph-mtk"])
+@pytest.mark.parametrize("model", ["corner8", "original"])
+def test_trajectory_parity(ensemble, model):
+    print("md parity", ensemble, model, flush=True)
+    s, verlet, omd = build((4, 3, 5), 0.3, 0.1, 31, ensemble, model)
+    compare(verlet, omd, "init")
+    for chunk in range(4):
+        verlet.run(25)
+        omd.run(25)
+        compare(verlet, omd, "step %d" % verlet.counter)
+    if "nhc" in ensemble:
+        ch = verlet.hooks[-1].thermostat.chain if "mtk" in ensemble else verlet.hooks[0].chain
+        assert np.abs(ch.vel - omd.chain.vel).max() < 1e-8 * np.abs(omd.chain.vel).max()
+        assert np.abs(ch.pos - omd.chain.pos).max() < 1e-8 * np.abs(omd.chain.pos).max()
+    if "mtk" in ensemble:
+        assert np.abs(verlet.rvecs - omd.rvecs).max() / ANG < 1e-8
+        assert np.abs(s.domain.rvecs - omd.rvecs).max() / ANG < 1e-8
+
+
+def test_nve_config0_1000_steps():
+    """Plain method, CPU-runnable case: 3^3 periodic grid, one type, NVE for 1000 steps."""
+    s, verlet, omd = build((3, 3, 3), 0.0, 0.0, 0, "nve", "corner8")
+    verlet.run(1000)
+    omd.run(1000)
+    compare(verlet, omd, "1000 steps")
+
+
+def test_pressure_part_in_md():
+    s, verlet, omd = build((3, 3, 3), 0.5, 0.0, 9, "npt-nhc-mtk", "corner8",
+                           ext_press=50 * units.bar, press=0.0)
+    verlet.run(40)
+    omd.run(40)
+    compare(verlet, omd, "pressure part")
+
+
+def test_graph_and_eager_agree_bitwise():
+    _, v1, _ = build((4, 4, 3), 0.3, 0.05, 5, "npt-nhc-mtk", "corner8")
+    _, v2, _ = build((4, 4, 3), 0.3, 0.05, 5, "npt-nhc-mtk", "corner8")
+    v2.use_graph = False
+    v1.run(30)
+    v2.run(30)
+    assert np.array_equal(v1.pos, v2.pos) and np.array_equal(v1.vel, v2.vel)
+    assert v1.econs == v2.econs
+
+
+class Recorder(Hook):
+    def __init__(self, start, step):
+        Hook.__init__(self, start, step)
+        self.seen = []
+
+    def __call__(self, iterative):
+        self.seen.append((iterative.counter, iterative.epot, iterative.rmsd_delta,
+                          iterative.mmf.system.pos.copy()))
+
+
+def test_host_hooks_cadence_and_state():
+    s = make_system((3, 4, 3), p_b=0.3, seed=3)
+    mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+    np.random.seed(3)
+    rec = Recorder(start=2, step=7)
+    verlet = VerletIntegrator(mmf, timestep=1.0 * FS, hooks=[rec, NHCThermostat(300.0)])
+    verlet.run(30)
+    assert [c for c, *_ in rec.seen] == [2, 9, 16, 23, 30]
+    # the state seen by the hook is the state after that step
+    np.random.seed(3)
+    s2 = make_system((3, 4, 3), p_b=0.3, seed=3)
+    mmf2 = MicMecForceField(s2, [ForcePartMechanical(s2)])
+    v2 = VerletIntegrator(mmf2, timestep=1.0 * FS, hooks=[NHCThermostat(300.0)])
+    for c, epot, rmsd_delta, pos in rec.seen:
+        while v2.counter < c:
+            prev = v2.pos.copy()
+            v2.propagate()
+        assert v2.epot == epot
+        assert np.array_equal(v2.pos, pos)
+        assert abs(rmsd_delta - np.sqrt(np.mean((pos - prev) ** 2))) < 1e-12 * rmsd_delta
+
+
+def test_nan_in_md_raises():
+    s = make_system((3, 3, 3), seed=4)
+    mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+    np.random.seed(4)
+    verlet = VerletIntegrator(mmf, timestep=1.0 * FS)
+    verlet._engine.pos[3, 0] = float("nan")
+    with pytest.raises(ValueError):
+        verlet.run(2)
