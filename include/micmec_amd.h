@@ -130,7 +130,7 @@ void mm_field_destroy(mm_field* f);
 #define MM_S_PTENS 88 /* 9 */
 #define MM_S_CE 97    /* 5: count, ekin mean, ekin var sum, econs mean, econs var sum */
 #define MM_S_CONS_ERR 102
-#define MM_S_VMAT 103 /* 9: pending velocity transform */
+#define MM_S_VMAT 103 /* 9: pending velocity matrix (applied after VSCALE_A, before VSCALE_B) */
 #define MM_S_VMAT_ON 112
 #define MM_S_XMAT 113 /* 9: pending position transform */
 #define MM_S_CHAINVEL0 122
@@ -139,7 +139,11 @@ void mm_field_destroy(mm_field* f);
 #define MM_S_RMSD_DELTA 125
 #define MM_S_NAN 126
 #define MM_S_EPOT_MECH 127
-#define MM_S_COUNT 128
+#define MM_S_VSCALE_A 128 /* pending velocity factor applied first (thermostat post) */
+#define MM_S_VSCALE_A_ON 129
+#define MM_S_VSCALE_B 130 /* pending velocity factor applied last (thermostat pre) */
+#define MM_S_VSCALE_B_ON 131
+#define MM_S_COUNT 144
 
 typedef struct mm_md mm_md;
 

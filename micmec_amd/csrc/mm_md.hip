@@ -93,18 +93,25 @@ __device__ __forceinline__ void vecmat(const double v[3], const double* M, doubl
   for (int j = 0; j < 3; ++j) out[j] = (v[0] * M[j] + v[1] * M[3 + j]) + v[2] * M[6 + j];
 }
 
+// Pending velocity transforms, applied in the order the reference performs them:
+// factor A (thermostat post of the previous half), matrix M (barostat), factor B (thermostat pre).
 __device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, double v[3]) {
 #pragma clang fp contract(off)
-  const int on = (int)S[MM_S_VMAT_ON];
-  if (on == 1) {
-    const double f = S[MM_S_VMAT];
+  if (S[MM_S_VSCALE_A_ON] != 0.0) {
+    const double f = S[MM_S_VSCALE_A];
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = v[j] * f;
-  } else if (on == 2) {
+  }
+  if (S[MM_S_VMAT_ON] != 0.0) {
     double t[3];
     vecmat(v, S + MM_S_VMAT, t);
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = t[j];
+  }
+  if (S[MM_S_VSCALE_B_ON] != 0.0) {
+    const double f = S[MM_S_VSCALE_B];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = v[j] * f;
   }
 }
 
@@ -403,30 +410,19 @@ __device__ void baro_part2(double* S) {
   A[0] += sh;
   A[4] += sh;
   A[8] += sh;
-  double R[9];
-  expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], R);
-  const int on = (int)S[MM_S_VMAT_ON];
-  if (on == 1) {
-    const double f = S[MM_S_VMAT];
-    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] = f * R[i];
-  } else if (on == 2) {
-    matmul3(S + MM_S_VMAT, R, S + MM_S_VMAT);
-  } else {
-    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] = R[i];
-  }
-  S[MM_S_VMAT_ON] = 2.0;
+  expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], S + MM_S_VMAT);
+  S[MM_S_VMAT_ON] = 1.0;
 }
 
-__device__ void set_vscale(double* S, double f) {
-  const int on = (int)S[MM_S_VMAT_ON];
-  if (on == 1) {
-    S[MM_S_VMAT] = S[MM_S_VMAT] * f;
-  } else if (on == 2) {
-    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] *= f;
-  } else {
-    S[MM_S_VMAT] = f;
-    S[MM_S_VMAT_ON] = 1.0;
-  }
+__device__ void clear_vtransforms(double* S) {
+  S[MM_S_VSCALE_A_ON] = 0.0;
+  S[MM_S_VMAT_ON] = 0.0;
+  S[MM_S_VSCALE_B_ON] = 0.0;
+}
+
+__device__ void set_vscale(double* S, int slot, double f) {
+  S[slot] = f;
+  S[slot + 1] = 1.0;
 }
 
 // Hook econs corrections as summed by compute_properties (verlet.py:176-181, npt.py:135-148).
@@ -546,7 +542,7 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       properties(S, false);
       break;
     case kPhNveEnd:
-      S[MM_S_VMAT_ON] = 0.0;
+      clear_vtransforms(S);
       take_force(S, f, true);
       take_nodes(S, nd);
       properties(S, true);
@@ -555,16 +551,16 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       double ekin = S[MM_S_EKIN];
       const double fac = nhc_chain(S, &ekin, false, 0.0);
       S[MM_S_EKIN] = ekin;
-      set_vscale(S, fac);
+      set_vscale(S, MM_S_VSCALE_B, fac);
       break;
     }
     case kPhNvtEnd: {
-      S[MM_S_VMAT_ON] = 0.0;
+      clear_vtransforms(S);
       take_force(S, f, true);
       take_nodes(S, nd);
       double ekin = S[MM_S_EKIN];
       const double fac = nhc_chain(S, &ekin, false, 0.0);
-      set_vscale(S, fac);
+      set_vscale(S, MM_S_VSCALE_A, fac);
       for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
       S[MM_S_EKIN] = ekin;
       S[MM_S_ECONS_THERMO] = nhc_econs(S);
@@ -576,8 +572,8 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       S[MM_S_HAS_CHAINVEL0] = thermo ? 1.0 : 0.0;
       S[MM_S_CHAINVEL0] = thermo ? S[MM_S_CHAIN_VEL] : 0.0;
       // a pending thermostat factor scales m v (x) v by its square
-      if ((int)S[MM_S_VMAT_ON] == 1) {
-        const double fac = S[MM_S_VMAT];
+      if (S[MM_S_VSCALE_A_ON] != 0.0) {
+        const double fac = S[MM_S_VSCALE_A];
         for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
       }
       baro_part1(S);
@@ -587,7 +583,7 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       baro_part2(S);
       break;
     case kPhBaroEndPre: {
-      S[MM_S_VMAT_ON] = 0.0;
+      clear_vtransforms(S);
       take_nodes(S, nd);
       update_baro_vel(S);
       if (thermo) {
@@ -595,12 +591,12 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
         double ekin = S[MM_S_EKIN];
         const double fac = nhc_chain(S, &ekin, true, g1);
         S[MM_S_EKIN] = ekin;
-        set_vscale(S, fac);
+        set_vscale(S, MM_S_VSCALE_B, fac);
       }
       break;
     }
     case kPhMainEndNpt: {
-      S[MM_S_VMAT_ON] = 0.0;
+      clear_vtransforms(S);
       take_force(S, f, true);
       take_nodes(S, nd);
       if (thermo) {
@@ -608,7 +604,7 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
         double ekin = S[MM_S_EKIN];
         const double fac = nhc_chain(S, &ekin, true, g1);
         S[MM_S_EKIN] = ekin;
-        set_vscale(S, fac);
+        set_vscale(S, MM_S_VSCALE_A, fac);
         for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
         S[MM_S_ECONS_THERMO] = nhc_econs(S);
       }
@@ -619,7 +615,7 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       break;
     }
     case kPhBaroEndPost:
-      S[MM_S_VMAT_ON] = 0.0;
+      clear_vtransforms(S);
       take_nodes(S, nd);
       update_baro_vel(S);
       S[MM_S_ECONS_BARO] = baro_ekin(S) + S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
@@ -632,7 +628,7 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       break;
     }
     case kPhApplied:
-      S[MM_S_VMAT_ON] = 0.0;
+      clear_vtransforms(S);
       break;
   }
 }

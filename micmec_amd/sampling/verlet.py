@@ -322,6 +322,11 @@ class VerletIntegrator(Iterative):
         self._S_version = getattr(self, "_S_version", 0) + 1
         if self._S[L.S_NAN] != 0.0:
             raise ValueError("The energy or the gradient became not-a-number (``nan``).")
+        if getattr(self, "_uploaded", False):
+            if self._baro is not None:
+                self.rvecs = self._S[L.S_RVECS:L.S_RVECS + 9].reshape(3, 3).copy()
+                self.mmf.system.domain.update_rvecs(self.rvecs)
+            self._mirror_hooks()
 
     def _s(self, i, n=None):
         if n is None:
@@ -422,6 +427,9 @@ class VerletIntegrator(Iterative):
             self._engine.call("mm_md_step", 1, int(bool(self.use_graph)))
         else:
             self._engine.call("mm_md_step", k, int(bool(self.use_graph)))
+        # a thermostat factor may still be pending on the velocities; apply it so that the
+        # device arrays hold the reference state between batches
+        self._engine.call("mm_md_sync")
         self.counter += k
         self._pull()
 
@@ -456,9 +464,6 @@ class VerletIntegrator(Iterative):
         self._engine.call("mm_md_rmsd")
         self._pull()
         self.mmf.system.pos[:] = self.pos
-        if self._baro is not None:
-            self.rvecs = self._s(L.S_RVECS, 9).reshape(3, 3)
-            self.mmf.system.domain.update_rvecs(self.rvecs)
         self._mirror_hooks()
 
     def _mirror_hooks(self):
