@@ -1,0 +1,212 @@
+#!/usr/bin/env python
+"""Throughput of FP64 micromechanical MD on one MI355X: cell-steps per second at 128^3 cells.
+
+Workload (``npt128``): periodic 128 x 128 x 128 grid, type B cells with probability 0.2
+(two metastable states), 5% empty cells, NPT at 300 K and 1 bar with a Nose-Hoover chain
+(length 3, 100 fs) and the MTK barostat (full 3x3 cell fluctuation, 1000 fs), dt = 1 fs, FP64,
+eight-corner nanocell model. Each step evaluates the force three times (MTK pre, main, MTK post).
+Cell-steps count grid cells n^3 (empty cells included) times steps. Inputs are synthetic and
+resident on the GPU before the timed region starts.
+
+Prints one JSON line. ``roofline`` refers to the dominant kernel (the per-cell force pass),
+timed with HIP events on the stream it is launched on; ``cpu_baseline`` times the numpy oracle
+on a bounded sample of the same workload (a 16^3 grid) on one host core.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 MD cell-steps/s @128³ cells; batched-replica vs plain speedup same GPU"
+PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) peak, FLOP/s
+PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
+# algorithmic FLOPs of the eight-corner model: per cell and state, per cell (edges, virial,
+# log-sum-exp), per node (gather and Verlet update)
+FLOP_CELL_STATE = 2290
+FLOP_CELL = 200
+FLOP_NODE = 54
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--model", default="corner8", choices=["corner8", "original"])
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--cpu-grid", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--kernel-launches", type=int, default=20)
+    return ap.parse_args()
+
+
+def build_workload(n, model):
+    import numpy as np
+    from micmec_amd import units
+    from micmec_amd.pes import ForcePartMechanical, MicMecForceField
+    from micmec_amd.sampling import MTKBarostat, NHCThermostat, VerletIntegrator
+    from micmec_amd.workloads import make_system
+    system = make_system((n, n, n), p_b=0.2, p_hole=0.05, seed=2)
+    fpm = ForcePartMechanical(system, model=model)
+    mmf = MicMecForceField(system, [fpm])
+    np.random.seed(2)
+    thermo = NHCThermostat(temp=300.0, timecon=100 * units.femtosecond, chainlength=3)
+    baro = MTKBarostat(mmf, temp=300.0, press=1.0 * units.bar,
+                       timecon=1000 * units.femtosecond)
+    verlet = VerletIntegrator(mmf, timestep=1.0 * units.femtosecond, hooks=[thermo, baro],
+                              temp0=300.0)
+    return system, fpm, verlet
+
+
+def kernel_timing(fpm, verlet, launches):
+    """Average duration of the cell pass, HIP events on the stream it is launched on."""
+    import ctypes
+    import torch
+    from micmec_amd import _native
+    lib = _native.lib()
+    eng = verlet._engine
+    stream = torch.cuda.current_stream()
+    sp = _native.stream_ptr(stream)
+    rv = eng.S[49:58]
+    for _ in range(3):
+        _native.check(lib.mm_field_launch_cells(fpm.field.handle, _native.ptr(eng.pos),
+                                                ctypes.c_void_p(rv.data_ptr()), sp))
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(launches):
+        _native.check(lib.mm_field_launch_cells(fpm.field.handle, _native.ptr(eng.pos),
+                                                ctypes.c_void_p(rv.data_ptr()), sp))
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1e3 / launches
+
+
+def algorithmic_counts(system, model):
+    """FLOPs and compulsory bytes of one cell pass (one force evaluation)."""
+    import numpy as np
+    from micmec_amd.pes import parse_type_params
+    type_ids, ns, *_ = parse_type_params(system.params, system.types)
+    per_type = dict(zip(type_ids, ns))
+    types = np.rint(np.asarray(system.types, dtype=float)).astype(int)
+    states = np.array([per_type[t] for t in types]) if len(types) else np.zeros(0)
+    scale = 1.0 if model == "corner8" else 1.0 / 7.0
+    flops = float(np.sum(FLOP_CELL + FLOP_CELL_STATE * scale * states))
+    # per cell: 8 corner indices + wrap + type (40 B); per node: position gathered once (24 B)
+    nbytes = 40.0 * len(types) + 24.0 * system.nnodes
+    return flops, nbytes, float(states.mean()) if len(states) else 0.0
+
+
+def read_traffic():
+    """HBM bytes per cell-pass launch from the committed rocprofv3 counter summary, if any."""
+    fn = os.path.join(ROOT, "profiles", "cell_kernel_npt128_traffic.txt")
+    if not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(n, seconds, model):
+    """Numpy oracle on an n^3 sample of the same workload, one host core."""
+    import numpy as np
+    from micmec_amd import units
+    from micmec_amd.topology import Grid
+    from micmec_amd.workloads import make_system
+    from oracle import mmff as omm
+    from oracle.integrators import OracleMD
+    s = make_system((n, n, n), p_b=0.2, p_hole=0.05, seed=2)
+    wb = Grid(np.rint(s.grid).astype(int)).wrap_bits()
+
+    def force(p, r):
+        return omm.compute_vectorised(p, r, s.types, s.surrounding_nodes, s.surrounding_cells,
+                                      wb, s.params, model)
+    np.random.seed(2)
+    md = OracleMD(force, s.pos, np.array(s.domain.rvecs), s.masses, units.femtosecond,
+                  thermo=dict(temp=300.0, timecon=100 * units.femtosecond),
+                  baro=dict(temp=300.0, press=units.bar, timecon=1000 * units.femtosecond))
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        md.propagate()
+        steps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": "numpy oracle (oracle/integrators.py + oracle/mmff.py), %d^3 grid of the "
+                      "same workload, %d NPT steps in %.1f s" % (n, steps, dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only; --gpus %d is not supported (this build measures "
+              "one GPU's share; multi-GPU sharding is a design note)." % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    n = args.grid
+    system, fpm, verlet = build_workload(n, args.model)
+    verlet.use_graph = not args.no_graph
+    print("workload built: %d cells (%d non-empty), %d nodes" % (n ** 3, system.ncells,
+                                                                 system.nnodes),
+          file=sys.stderr, flush=True)
+    verlet.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    verlet.run(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = n ** 3 * args.steps / elapsed
+    print("timed %d steps in %.3f s" % (args.steps, elapsed), file=sys.stderr, flush=True)
+
+    t_cell = kernel_timing(fpm, verlet, args.kernel_launches)
+    flops, nbytes, mean_states = algorithmic_counts(system, args.model)
+    achieved = flops / t_cell
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": read_traffic(),
+                "kernel": "cell_kernel<corner8>" if args.model == "corner8" else
+                          "cell_kernel<original>",
+                "avg_launch_ms": t_cell * 1e3, "flop_per_launch": flops,
+                "compute_unit": "FP64 VALU (no MFMA; 78.6 TFLOP/s is the MI355X FP64 peak)"}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "npt128", "grid": [n, n, n], "cells_nonempty": system.ncells,
+                   "nodes": system.nnodes, "mean_states": mean_states,
+                   "ensemble": "NPT NHC(3)+MTK, 300 K, 1 bar", "timestep_fs": 1.0,
+                   "force_evals_per_step": 3, "nanocell": args.model,
+                   "parallelism": "single GPU", "graph": not args.no_graph},
+        "roofline": roofline,
+    }
+    if args.cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_grid, args.cpu_seconds, args.model)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

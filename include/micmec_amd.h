@@ -74,6 +74,10 @@ int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double
  * (micmec/pes/mmff.py:135-148). */
 int mm_field_check(mm_field* f, void* stream, int32_t* flags);
 
+/* The cell pass of mm_field_compute alone (per-cell energies, gradients and virial partials
+ * into the object's own buffers), for per-kernel timing and profiling; no output is written. */
+int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, void* stream);
+
 /* Sizes of the object: cells, nodes, padded work slots. */
 int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* nslots);
 

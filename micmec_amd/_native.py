@@ -48,6 +48,7 @@ def _load():
                                   P, P, i32, i32, P, P, P, P, P, f64]
     L.mm_field_compute.argtypes = [P, P, P, P, P, P, ctypes.c_int, P]
     L.mm_field_check.argtypes = [P, P, ctypes.POINTER(i32)]
+    L.mm_field_launch_cells.argtypes = [P, P, P, P]
     L.mm_field_info.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.mm_field_destroy.argtypes = [P]
     L.mm_field_destroy.restype = None

@@ -320,6 +320,37 @@ int mm_field_check(mm_field* f, void* stream, int32_t* flags) {
   return MM_OK;
 }
 
+int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, void* stream) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if ((f->N > 0 && !pos) || !rvecs) return set_error(MM_ERR_ARG, "pos or rvecs is NULL");
+  DeviceGuard guard(f->device);
+  ForceArgs a;
+  a.pos = pos;
+  a.rvecs = rvecs;
+  a.cnode = f->cnode;
+  a.cwrap = f->cwrap;
+  a.blk_row = f->blk_row;
+  a.tab = f->tab;
+  a.kt = f->kt;
+  a.nstates = f->nstates;
+  a.max_states = f->max_states;
+  a.W = f->W;
+  a.N = f->N;
+  a.ncell = f->ncell;
+  a.gcell = f->gcell;
+  a.partials = f->partials;
+  a.energy = nullptr;
+  a.gpos = nullptr;
+  a.vtens = nullptr;
+  a.accumulate = 0;
+  a.model = f->model;
+  a.nanflag = f->nanflag;
+  a.force_grad = 1;
+  hipError_t e = launch_cells(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_error(e, "cell kernel");
+  return MM_OK;
+}
+
 int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* nslots) {
   if (!f) return set_error(MM_ERR_ARG, "field is NULL");
   if (ncells) *ncells = f->C;
