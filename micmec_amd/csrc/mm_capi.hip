@@ -171,7 +171,7 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
   if (ncells < 0 || nnodes < 0) return set_error(MM_ERR_ARG, "negative sizes");
   if (ncells > (1LL << 27)) return set_error(MM_ERR_ARG, "too many cells (%lld)", (long long)ncells);
   if (nnodes > (1LL << 30)) return set_error(MM_ERR_ARG, "too many nodes");
-  if (nrows < 1 || nrows > 65535 || max_states < 1 || max_states > 64)
+  if (nrows < 1 || nrows > 65535 || max_states < 1 || max_states > kMaxStates)
     return set_error(MM_ERR_ARG, "bad parameter table shape %d x %d", nrows, max_states);
   if (ncells > 0 && (!surrounding_nodes || !wrap_bits || !cell_row))
     return set_error(MM_ERR_ARG, "missing cell tables");
@@ -218,7 +218,7 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
       const int64_t c = by_row[t][i];
       const int64_t w = w0 + (int64_t)i;
       slot[c] = w;
-      for (int k = 0; k < 8; ++k) cnode[(size_t)k * W + w] = (int32_t)surrounding_nodes[c * 8 + k];
+      for (int k = 0; k < 8; ++k) cnode[(size_t)w * 8 + k] = (int32_t)surrounding_nodes[c * 8 + k];
       cwrap[w] = wrap_bits[c];
     }
     w0 += nb * kCellBlock;

@@ -91,8 +91,9 @@ __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, 
 // kModel: 0 = corner8 (nanocell.py), 1 = original (nanocell_original.py).
 template <int kModel, bool kGrad>
 __device__ __forceinline__ double state_energy(const double* __restrict__ tab, const double r[8][3],
-                                               double g[8][3]) {
+                                               double g[8][3], double gscale = 1.0) {
   const double w = tab[9];
+  const double wg = w * gscale;
   if (kModel == 0) {
     double usum = 0.0;
 #pragma unroll
@@ -109,7 +110,7 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
         for (int d = 0; d < 3; ++d) {
 #pragma unroll
           for (int j = 0; j < 3; ++j) {
-            const double v = w * G[3 * d + j];
+            const double v = wg * G[3 * d + j];
             g[kHi[a][d]][j] += v;
             g[kLo[a][d]][j] -= v;
           }
@@ -138,12 +139,53 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
         for (int d = 0; d < 3; ++d) {
           const double sgn = kOff[v][d] ? 0.25 : -0.25;
 #pragma unroll
-          for (int j = 0; j < 3; ++j) g[v][j] += sgn * w * G[3 * d + j];
+          for (int j = 0; j < 3; ++j) g[v][j] += sgn * wg * G[3 * d + j];
         }
       }
     }
     return w * u;
   }
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += __shfl_down(v[q], off, 64);
+  }
+}
+
+// Sum SoA block partials part[q * nb + b] over b in a fixed order; one block of 1024 threads,
+// every thread receives the totals.
+template <int NV>
+__device__ __forceinline__ void reduce_soa(const double* __restrict__ part, int nb,
+                                           double (&out)[NV]) {
+  __shared__ double red[NV * 16];
+  __shared__ double tot[NV];
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (int bb = threadIdx.x; bb < nb; bb += 1024) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += part[(int64_t)q * nb + bb];
+  }
+  wave_reduce<NV>(acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave * NV + q] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = red[threadIdx.x];
+    for (int w = 1; w < 16; ++w) s += red[w * NV + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[q] = tot[q];
+  __syncthreads();
 }
 
 }  // namespace mm

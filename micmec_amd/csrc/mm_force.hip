@@ -2,16 +2,21 @@
 //
 // Reference path: ForcePartMechanical._internal_compute (micmec/pes/mmff.py:288-323) and
 // deformation (mmff.py:326-403). Layout in HBM (all FP64 unless noted):
-//   pos      [N][3]            node positions (caller's buffer)
-//   rvecs    [3][3]            domain vectors, rows (device copy)
-//   cnode    [8][W] int32      corner k of work slot w -> node (SoA: coalesced per corner)
-//   cwrap    [W]    int32      wrap bits of the cell in slot w
-//   blk_row  [W/256] int32     parameter-table row shared by all cells of a block
-//   ncell    [8][N] int32      node n -> 8*w + k of the cell in which n is corner k, -1 none
-//   gcell    [8][3][W]         per-(corner, component) vertex gradient of each cell
-//   partials [W/256][16]       per-block energy (0) and virial (1..9)
-// Cells are grouped by type into block-aligned segments, so the type, its number of states and
-// its parameter row are uniform per block and the parameters are read with scalar loads.
+//   pos      [N][3]              node positions (caller's buffer)
+//   rvecs    [3][3]              domain vectors, rows (device copy)
+//   cnode    [W][8] int32        node at corner k of work slot w (-1 = padding slot)
+//   cwrap    [W]    int32        wrap bits of the cell in slot w
+//   blk_row  [W/256] int32       parameter-table row shared by all cells of a block
+//   ncell    [8][N] int32        node n -> 8*w + k of the cell in which n is corner k, -1 none
+//   gcell    [8][3][W]           vertex gradient of every corner of every cell (SoA)
+//   partials [16][W/256]         per-block energy (0) and virial (1..9), SoA
+// Cells are grouped by type into 256-slot segments, so the type, its number of states and its
+// parameter row are uniform per block and the parameters are read with scalar loads.
+//
+// One lane per cell: the lane gathers the cell's eight nodes, evaluates the corner matrices of
+// every state and accumulates the eight vertex gradients in registers. Blocks of single-state
+// types and of multistate types run in separate launches so that the single-state kernel does
+// not carry the registers of the state combination.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -20,18 +25,11 @@
 
 namespace mm {
 
-template <int NV>
-__device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] += __shfl_down(v[q], off, 64);
-  }
-}
-
-// Deterministic block reduction: fixed shuffle tree inside each wave, then waves in order.
+// Deterministic block reduction (fixed shuffle tree per wave, waves in order); the block's
+// totals are written SoA to out[q * stride].
 template <int NV, int BS>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], double* smem) {
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* smem, double* out,
+                                                   int64_t stride) {
   wave_reduce<NV>(v);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -40,13 +38,11 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* smem) {
     for (int q = 0; q < NV; ++q) smem[wave * NV + q] = v[q];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < NV) {
+    double s = smem[threadIdx.x];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = smem[q];
-      for (int w = 1; w < BS / 64; ++w) s += smem[w * NV + q];
-      v[q] = s;
-    }
+    for (int w = 1; w < BS / 64; ++w) s += smem[w * NV + threadIdx.x];
+    out[threadIdx.x * stride] = s;
   }
 }
 
@@ -78,8 +74,8 @@ __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
 // One lane per cell. Energies of all states of the block's cell type, combined with a running
 // log-sum-exp (mmff.py:386-398); the vertex gradient of the combined energy goes to gcell and
 // the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials.
-template <int kModel, bool kGrad>
-__global__ void __launch_bounds__(kCellBlock)
+template <int kModel, bool kGrad, bool kMulti>
+__global__ void __launch_bounds__(kCellBlock, 2)
 cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             const int32_t* __restrict__ cnode, const int32_t* __restrict__ cwrap,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
@@ -87,7 +83,10 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             int32_t max_states, int64_t W, double* __restrict__ gcell,
             double* __restrict__ partials) {
   __shared__ double red[10 * (kCellBlock / 64)];
+  __shared__ double es_sh[kMulti ? kMaxStates * kCellBlock : 1];
   const int trow = blk_row[blockIdx.x];
+  // single-state and multistate blocks run in separate launches (see launch_cells)
+  if (kMulti != (nstates_g[trow] > 1)) return;
   const int ns = nstates_g[trow];
   const double kt = kt_g[trow];
   const double* tab = tab_g + (int64_t)trow * max_states * kTabStride;
@@ -98,7 +97,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   const int64_t w = (int64_t)blockIdx.x * kCellBlock + threadIdx.x;
   int n[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) n[k] = cnode[k * W + w];
+  for (int k = 0; k < 8; ++k) n[k] = cnode[w * 8 + k];
   double acc[10];
 #pragma unroll
   for (int q = 0; q < 10; ++q) acc[q] = 0.0;
@@ -111,41 +110,30 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
     for (int v = 0; v < 8; ++v)
 #pragma unroll
       for (int j = 0; j < 3; ++j) gacc[v][j] = 0.0;
-    double emin = tab[10] + state_energy<kModel, kGrad>(tab, r, gacc);
-    double zsum = 1.0;
-    for (int s = 1; s < ns; ++s) {
-      const double* ts = tab + s * kTabStride;
-      double gs[8][3];
-#pragma unroll
-      for (int v = 0; v < 8; ++v)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) gs[v][j] = 0.0;
-      const double es = ts[10] + state_energy<kModel, kGrad>(ts, r, gs);
-      if (es < emin) {
-        const double f = exp(-(emin - es) / kt);
-        zsum = zsum * f + 1.0;
-#pragma unroll
-        for (int v = 0; v < 8; ++v)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) gacc[v][j] = gacc[v][j] * f + gs[v][j];
-        emin = es;
-      } else {
-        const double f = exp(-(es - emin) / kt);
-        zsum += f;
-#pragma unroll
-        for (int v = 0; v < 8; ++v)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) gacc[v][j] += f * gs[v][j];
+    double ecell;
+    if (!kMulti) {
+      ecell = tab[10] + state_energy<kModel, kGrad>(tab, r, gacc);
+    } else {
+      // pass 1: energies of all states, E = E_min - kT ln sum exp(-(E_s - E_min)/kT)
+      // (mmff.py:386-395); the state energies are kept in LDS
+      double emin = 0.0;
+      for (int s = 0; s < ns; ++s) {
+        const double* ts = tab + s * kTabStride;
+        const double es = ts[10] + state_energy<kModel, false>(ts, r, gacc);
+        es_sh[s * kCellBlock + threadIdx.x] = es;
+        emin = (s == 0 || es < emin) ? es : emin;
       }
-    }
-    double ecell = emin;
-    if (ns > 1) {
+      double zsum = 0.0;
+      for (int s = 0; s < ns; ++s) zsum += exp(-(es_sh[s * kCellBlock + threadIdx.x] - emin) / kt);
       ecell = emin - kt * log(zsum);
-      const double inv = 1.0 / zsum;
-#pragma unroll
-      for (int v = 0; v < 8; ++v)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) gacc[v][j] *= inv;
+      // pass 2: gradient sum_s p_s dE_s/dr with p_s = exp(-(E_s - E_min)/kT) / Z (mmff.py:396-398)
+      if (kGrad) {
+        for (int s = 0; s < ns; ++s) {
+          const double* ts = tab + s * kTabStride;
+          const double ps = exp(-(es_sh[s * kCellBlock + threadIdx.x] - emin) / kt) / zsum;
+          (void)state_energy<kModel, true>(ts, r, gacc, ps);
+        }
+      }
     }
     acc[0] = ecell;
     if (kGrad) {
@@ -166,15 +154,10 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       }
     }
   }
-  block_reduce<10, kCellBlock>(acc, red);
-  if (threadIdx.x == 0) {
-    double* p = partials + (int64_t)blockIdx.x * kPartialStride;
-#pragma unroll
-    for (int q = 0; q < 10; ++q) p[q] = acc[q];
-  }
+  block_reduce_store<10, kCellBlock>(acc, red, partials + blockIdx.x, gridDim.x);
 }
 
-// gpos[n] (+)= sum_{k=0..7} gcell[k][:, w(n,k)] in fixed k order (mmff.py:303-318).
+// gpos[n] (+)= sum_{k=0..7} gcell[k][:][w(n,k)] in fixed k order (mmff.py:303-318).
 __global__ void __launch_bounds__(256)
 gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell, int64_t N,
               int64_t W, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag) {
@@ -200,19 +183,11 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
   if (bad) atomicOr(nanflag, 2);
 }
 
-// Sum block partials in fixed order; write energy / virial (overwrite or accumulate).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 finalize_kernel(const double* __restrict__ partials, int nblocks, double* __restrict__ energy,
                 double* __restrict__ vtens, int accumulate, int* __restrict__ nanflag) {
-  __shared__ double red[10 * 4];
   double acc[10];
-#pragma unroll
-  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += 256) {
-#pragma unroll
-    for (int q = 0; q < 10; ++q) acc[q] += partials[(int64_t)b * kPartialStride + q];
-  }
-  block_reduce<10, 256>(acc, red);
+  reduce_soa<10>(partials, nblocks, acc);
   if (threadIdx.x == 0) {
     int bad = 0;
     if (acc[0] != acc[0]) bad |= 1;
@@ -231,14 +206,18 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   const int nb = (int)(a.W / kCellBlock);
   const bool grad = a.gpos != nullptr || a.vtens != nullptr || a.force_grad;
   if (nb == 0) return hipSuccess;
-#define MM_LAUNCH_CELL(MODEL, GRAD)                                                            \
-  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD>), dim3(nb), dim3(kCellBlock), 0, stream, a.pos, \
-                     a.rvecs, a.cnode, a.cwrap, a.blk_row, a.tab, a.kt, a.nstates,             \
-                     a.max_states, a.W, a.gcell, a.partials)
+  // Both launches cover every block; each block returns at once unless its type matches the
+  // launch, so the block partials are written exactly once per block.
+#define MM_LAUNCH_CELL(MODEL, GRAD, MULTI)                                                   \
+  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(nb), dim3(kCellBlock), 0,       \
+                     stream, a.pos, a.rvecs, a.cnode, a.cwrap, a.blk_row, a.tab, a.kt,       \
+                     a.nstates, a.max_states, a.W, a.gcell, a.partials)
   if (a.model == 0) {
-    if (grad) MM_LAUNCH_CELL(0, true); else MM_LAUNCH_CELL(0, false);
+    if (grad) { MM_LAUNCH_CELL(0, true, false); MM_LAUNCH_CELL(0, true, true); }
+    else { MM_LAUNCH_CELL(0, false, false); MM_LAUNCH_CELL(0, false, true); }
   } else {
-    if (grad) MM_LAUNCH_CELL(1, true); else MM_LAUNCH_CELL(1, false);
+    if (grad) { MM_LAUNCH_CELL(1, true, false); MM_LAUNCH_CELL(1, true, true); }
+    else { MM_LAUNCH_CELL(1, false, false); MM_LAUNCH_CELL(1, false, true); }
   }
 #undef MM_LAUNCH_CELL
   return hipGetLastError();
@@ -250,12 +229,12 @@ hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
   if (err != hipSuccess) return err;
   if (a.gpos != nullptr && a.N > 0 && nb > 0) {
     const int gb = (int)((a.N + 255) / 256);
-    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N, a.W,
-                       a.gpos, a.accumulate, a.nanflag);
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
+                       a.W, a.gpos, a.accumulate, a.nanflag);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, stream, a.partials, nb, a.energy,
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, stream, a.partials, nb, a.energy,
                      a.vtens, a.accumulate, a.nanflag);
   return hipGetLastError();
 }

@@ -8,11 +8,12 @@ namespace mm {
 
 constexpr int kCellBlock = 256;
 constexpr int kPartialStride = 16;
+constexpr int kMaxStates = 8;  // metastable states per cell type
 
 struct ForceArgs {
   const double* pos;       // [N][3]
   const double* rvecs;     // [3][3] device
-  const int32_t* cnode;    // [8][W] corner nodes in work order (-1 = padding slot)
+  const int32_t* cnode;    // [W][8] corner nodes in work order (-1 = padding slot)
   const int32_t* cwrap;    // [W] wrap bits of each work slot
   const int32_t* blk_row;  // [W / kCellBlock] table row of every block
   const double* tab;       // [rows][max_states][kTabStride]
@@ -23,7 +24,7 @@ struct ForceArgs {
   int64_t N;
   const int32_t* ncell;    // [8][N] node -> 8*w + k, -1 = none
   double* gcell;           // [8][3][W]
-  double* partials;        // [W / kCellBlock][kPartialStride]
+  double* partials;        // [kPartialStride][W / kCellBlock] (SoA)
   double* energy;          // [1] or null
   double* gpos;            // [N][3] or null
   double* vtens;           // [9] or null

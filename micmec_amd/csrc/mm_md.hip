@@ -45,32 +45,21 @@ enum Phase {
 
 // ----------------------------------------------------------------------------- node kernels
 
+// Block totals of a node pass, written SoA: out[q * gridDim.x + blockIdx.x].
 template <int NV>
-__device__ __forceinline__ void wave_sum(double (&v)[NV]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] += __shfl_down(v[q], off, 64);
-  }
-}
-
-template <int NV>
-__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* out) {
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* part) {
   __shared__ double red[NV * (kNodeBlock / 64)];
-  wave_sum<NV>(v);
+  wave_reduce<NV>(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) red[wave * NV + q] = v[q];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = red[q];
-      for (int w = 1; w < kNodeBlock / 64; ++w) s += red[w * NV + q];
-      out[q] = s;
-    }
+  if (threadIdx.x < NV) {
+    double s = red[threadIdx.x];
+    for (int w = 1; w < kNodeBlock / 64; ++w) s += red[w * NV + threadIdx.x];
+    part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -168,7 +157,7 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
     }
   }
   if (kMode == kScaleVel || kMode == kVV || kMode == kRmsd) {
-    block_sum_store<13>(acc, partials + (int64_t)blockIdx.x * kNodePartial);
+    block_sum_store<13>(acc, partials);
   }
 }
 
@@ -215,7 +204,7 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       node_vv(vv, mass, acc);
     }
   }
-  block_sum_store<13>(acc, partials + (int64_t)blockIdx.x * kNodePartial);
+  block_sum_store<13>(acc, partials);
 }
 
 // ----------------------------------------------------------------------------- scalar math
@@ -497,30 +486,7 @@ __device__ void take_nodes(double* S, const double (&nd)[13]) {
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
 }
 
-template <int NV>
-__device__ void reduce_partials(const double* __restrict__ part, int nb, int stride,
-                                double (&out)[NV]) {
-  __shared__ double red[NV * 4];
-  double acc[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] += part[(int64_t)b * stride + q];
-  }
-  wave_sum<NV>(acc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) red[wave * NV + q] = acc[q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) out[q] = ((red[q] + red[NV + q]) + red[2 * NV + q]) + red[3 * NV + q];
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpart, int nbf,
               const double* __restrict__ npart, int nbn, int with_vtens, int64_t N) {
   double f[10], nd[13];
@@ -529,8 +495,8 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
   const bool use_n = phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
                      phase == kPhBaroEndPre || phase == kPhMainEndNpt ||
                      phase == kPhBaroEndPost || phase == kPhRmsd;
-  if (use_f) reduce_partials<10>(fpart, nbf, kPartialStride, f);
-  if (use_n) reduce_partials<13>(npart, nbn, kNodePartial, nd);
+  if (use_f) reduce_soa<10>(fpart, nbf, f);
+  if (use_n) reduce_soa<13>(npart, nbn, nd);
   if (threadIdx.x != 0) return;
   const bool thermo = S[MM_S_THERMO] != 0.0;
   switch (phase) {
@@ -714,7 +680,7 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
 
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
   const int nbf = (int)(md->field->W / kCellBlock);
-  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(256), 0, s, phase, md->S,
+  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(1024), 0, s, phase, md->S,
                      md->field->partials, nbf, md->npart, md->nbn, with_vtens, md->N);
   return hipGetLastError();
 }
