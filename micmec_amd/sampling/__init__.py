@@ -6,9 +6,12 @@ from .iterative import (Iterative, StateItem, AttributeStateItem, PosStateItem,
 from .verlet import VerletIntegrator, VerletHook, VerletScreenLog, ConsErrTracker
 from .nvt import NHChain, NHCThermostat, NHCAttributeStateItem
 from .npt import TBCombination, MTKBarostat, MTKAttributeStateItem
+from .dof import DOF, CartesianDOF
+from .opt import OptScreenLog, BaseOptimizer, CGOptimizer, ConjugateGradientSearch
 
 __all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
            "TemperatureStateItem", "VolumeStateItem", "DomainStateItem", "ConsErrStateItem",
            "EPotContribStateItem", "Hook", "VerletIntegrator", "VerletHook", "VerletScreenLog",
            "ConsErrTracker", "NHChain", "NHCThermostat", "NHCAttributeStateItem",
-           "TBCombination", "MTKBarostat", "MTKAttributeStateItem"]
+           "TBCombination", "MTKBarostat", "MTKAttributeStateItem", "DOF", "CartesianDOF",
+           "OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch"]

@@ -1,0 +1,186 @@
+"""Geometry optimisation (drop-in for micmec/sampling/opt.py:46-180).
+
+``CGOptimizer`` keeps the reference API (``CGOptimizer(dof, state, hooks, counter0)``, ``run``,
+``propagate`` returning convergence, ``OptScreenLog``). The reference delegates the search to
+molmod's ``Minimizer(ConjugateGradient(), NewtonLineSearch())`` (opt.py:153-162), which is not
+part of this build; this module implements its own Polak-Ribiere (PR+) conjugate gradient with
+a Newton/secant line search on the directional derivative. When the degrees of freedom are
+device-capable, x, gradient and search direction live on the GPU and every energy/gradient is
+one launch of the HIP force kernels; only scalars return to the host.
+"""
+
+import time
+
+import numpy as np
+import torch
+
+from ..log import log
+from .iterative import (AttributeStateItem, DomainStateItem, EPotContribStateItem, Hook,
+                        Iterative, PosStateItem, VolumeStateItem)
+
+__all__ = ["OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch"]
+
+
+class OptScreenLog(Hook):
+    def __init__(self, start=0, step=1):
+        Hook.__init__(self, start, step)
+        self.time0 = None
+
+    def expects_call(self, counter):
+        return log.do_medium and Hook.expects_call(self, counter)
+
+    def __call__(self, iterative):
+        if self.time0 is None:
+            self.time0 = time.time()
+            log.hline()
+            log("counter  Conv.val.  N           Worst     Energy   Walltime")
+            log.hline()
+        log("%7i % 10.3e %2i %15s % 14.8e %10.1f" % (
+            iterative.counter, iterative.dof.conv_val, iterative.dof.conv_count,
+            iterative.dof.conv_worst, iterative.epot, time.time() - self.time0))
+
+
+class BaseOptimizer(Iterative):
+    default_state = [
+        AttributeStateItem("counter"),
+        AttributeStateItem("epot"),
+        PosStateItem(),
+        VolumeStateItem(),
+        DomainStateItem(),
+        EPotContribStateItem(),
+    ]
+    log_name = "XXOPT"
+
+    def __init__(self, dof, state=None, hooks=None, counter0=0):
+        self.dof = dof
+        Iterative.__init__(self, dof.mmf, state, hooks, counter0)
+
+    def _add_default_hooks(self):
+        if not any(isinstance(hook, OptScreenLog) for hook in self.hooks):
+            self.hooks.append(OptScreenLog())
+
+    def fun(self, x, do_gradient=False):
+        if do_gradient:
+            self.epot, gx = self.dof.fun(x, True)
+            return self.epot, gx
+        self.epot = self.dof.fun(x, False)
+        return self.epot
+
+    def initialize(self):
+        self.dof.check_convergence()
+        Iterative.initialize(self)
+
+    def propagate(self):
+        self.dof.check_convergence()
+        Iterative.propagate(self)
+        return self.dof.converged
+
+
+def _dot(a, b):
+    if isinstance(a, torch.Tensor):
+        return float(torch.dot(a, b).item())
+    return float(np.dot(a, b))
+
+
+class ConjugateGradientSearch(object):
+    """Polak-Ribiere (PR+) conjugate gradient with a secant line search on phi'(alpha).
+
+    One ``propagate`` is one line search and one direction update, the unit of one iteration
+    of the reference's minimiser.
+    """
+
+    def __init__(self, x0, fun, max_ls=20, curvature=1e-3, max_step=None):
+        self.x = x0
+        self.fun = fun
+        self.max_ls = max_ls
+        self.curvature = curvature
+        self.max_step = max_step
+
+    def initialize(self):
+        self.f, self.g = self.fun(self.x, True)
+        self.d = -self.g
+        gnorm = np.sqrt(_dot(self.g, self.g))
+        self.alpha = 1.0 / gnorm if gnorm > 0 else 1.0
+        self.restarts = 0
+
+    def propagate(self):
+        dphi0 = _dot(self.g, self.d)
+        if not dphi0 < 0.0:
+            # not a descent direction: restart along the steepest descent
+            self.d = -self.g
+            dphi0 = _dot(self.g, self.d)
+            self.restarts += 1
+            if dphi0 == 0.0:
+                return True
+        f0 = self.f
+        ftol = 8.0 * np.finfo(float).eps * max(abs(f0), np.finfo(float).tiny)
+        a_lo, dp_lo = 0.0, dphi0
+        a = self.alpha
+        best = None
+        for _ in range(self.max_ls):
+            xa = self.x + a * self.d
+            fa, ga = self.fun(xa, True)
+            dp = _dot(ga, self.d)
+            ok = np.isfinite(fa) and fa <= f0 + ftol
+            if ok and (best is None or abs(dp) < abs(best[4])):
+                best = (a, fa, ga, xa, dp)
+            if ok and abs(dp) <= self.curvature * abs(dphi0):
+                break
+            # secant step on phi'(alpha) through the two latest points; exact on quadratics
+            if np.isfinite(fa) and dp > dp_lo:
+                a_new = a - dp * (a - a_lo) / (dp - dp_lo)
+            elif np.isfinite(fa) and dp < 0.0:
+                a_new = 2.0 * a
+            else:
+                a_new = 0.5 * (a + a_lo)
+            if not np.isfinite(a_new) or a_new <= 0.0:
+                a_new = 0.5 * a
+            if self.max_step is not None:
+                a_new = min(a_new, self.max_step)
+            a_lo, dp_lo = a, dp
+            a = a_new
+        if best is None:
+            return False
+        a, fa, ga, xa, _ = best
+        g_old = self.g
+        self.x, self.f, self.g = xa, fa, ga
+        gg = _dot(g_old, g_old)
+        beta = max(0.0, _dot(ga, ga - g_old) / gg) if gg > 0 else 0.0
+        self.d = -ga + beta * self.d
+        self.alpha = a
+        return True
+
+
+class CGOptimizer(BaseOptimizer):
+    """Conjugate-gradient optimiser (opt.py:147-180)."""
+    log_name = "CGOPT"
+
+    def __init__(self, dof, state=None, hooks=None, counter0=0, device=True):
+        x0 = dof.x0
+        if device and getattr(dof, "device", None) is not None:
+            x0 = torch.as_tensor(np.ascontiguousarray(x0), dtype=torch.float64,
+                                 device=dof.device)
+        self.minimizer = ConjugateGradientSearch(x0, self.fun)
+        BaseOptimizer.__init__(self, dof, state, hooks, counter0)
+
+    def initialize(self):
+        self.minimizer.initialize()
+        BaseOptimizer.initialize(self)
+
+    def propagate(self):
+        success = self.minimizer.propagate()
+        self.x = self.minimizer.x
+        # make the accepted point the current state of the degrees of freedom
+        self.epot = self.minimizer.f
+        if isinstance(self.x, torch.Tensor):
+            self.dof._pos_dev = self.x.view(-1, 3) if self.dof.select is None else \
+                self.dof._pos_dev
+            self.dof._gpos_dev = self.minimizer.g.view(-1, 3) if self.dof.select is None else \
+                self.dof._gpos_dev
+        else:
+            self.dof._update(self.x)
+            self.dof._gpos[:] = self.minimizer.g.reshape(-1, 3)
+        if not success:
+            log.warn("Line search failed in optimizer. Aborting optimization.")
+            return True
+        return BaseOptimizer.propagate(self)

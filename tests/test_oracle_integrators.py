@@ -1,0 +1,134 @@
+"""Pins of the CPU oracle integrator (oracle/integrators.py).
+
+* NVE: energy conservation and exact time reversibility of velocity Verlet;
+* NHC: the extended energy of the chain (nvt.py:453-458) is conserved once the chain's
+  correction is active (the reference reports econs without it before the first post hook);
+* the chain's half-step operator follows the Nose-Hoover-chain equations (high-accuracy ODE);
+* MTK: the conserved quantity converges with the time step, and MTK at pressure P equals MTK at
+  pressure 0 plus the external-pressure part P.V (npt.py:645-672);
+* seeded initial states follow numpy's legacy stream in the reference's draw order.
+"""
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from micmec_amd import units
+from micmec_amd.topology import Grid
+from micmec_amd.workloads import make_system
+from oracle import mmff as omm
+from oracle.integrators import NHChain, OracleMD, get_random_vel
+
+FS = units.femtosecond
+
+
+def _force(s, model="corner8"):
+    wb = Grid(np.rint(s.grid).astype(int)).wrap_bits()
+
+    def f(p, r):
+        return omm.compute_vectorised(p, r, s.types, s.surrounding_nodes, s.surrounding_cells,
+                                      wb, s.params, model)
+    return f
+
+
+@pytest.fixture(scope="module")
+def small():
+    return make_system((3, 4, 3), p_b=0.3, p_hole=0.1, seed=31)
+
+
+def test_nve_conservation_and_reversibility(small):
+    s = small
+    np.random.seed(1)
+    md = OracleMD(_force(s), s.pos, np.array(s.domain.rvecs), s.masses, FS)
+    x0, e0 = md.pos.copy(), md.econs
+    md.run(100)
+    assert abs(md.econs - e0) < 1e-5 * md.ekin
+    md.vel = -md.vel
+    md.gpos = _force(s)(md.pos, md.rvecs)[1]
+    md.run(100)
+    assert np.abs(md.pos - x0).max() < 1e-9
+
+
+def test_nhc_extended_energy_conserved(small):
+    s = small
+    np.random.seed(2)
+    md = OracleMD(_force(s), s.pos, np.array(s.domain.rvecs), s.masses, FS,
+                  thermo=dict(temp=300.0, timecon=100 * FS))
+    md.propagate()
+    e1 = md.econs
+    for _ in range(150):
+        md.propagate()
+    assert abs(md.econs - e1) < 2e-4 * md.ekin
+
+
+def test_chain_operator_follows_ode():
+    """The chain half-step operator is a second-order splitting of the NHC equations."""
+    kt = units.boltzmann * 300.0
+    ndof = 147
+    np.random.seed(3)
+    ch0 = NHChain(2, 1.0 * FS, 300.0, 100 * FS)
+    ch0.set_ndof(ndof)
+    Q = ch0.masses.copy()
+    v0 = ch0.vel.copy()
+    ke0 = 0.07
+
+    def rhs(t, y):
+        ke, x0, x1, a, b = y
+        return [-2 * a * ke, a, b, (2 * ke - ndof * kt) / Q[0] - a * b, (Q[0] * a * a - kt) / Q[1]]
+    T = 50 * FS
+    ref = solve_ivp(rhs, [0, T], [ke0, 0, 0, v0[0], v0[1]], rtol=1e-12, atol=1e-20).y[:, -1]
+    errs = []
+    for dt in (1.0 * FS, 0.5 * FS):
+        ch = NHChain(2, dt, 300.0, 100 * FS)
+        ch.ndof, ch.masses, ch.vel = ndof, Q.copy(), v0.copy()
+        ke, vel = ke0, np.ones(1)
+        for _ in range(int(round(T / dt))):
+            vel, ke = ch(ke, vel, None)
+            vel, ke = ch(ke, vel, None)
+        errs.append(abs(ke - ref[0]) / ref[0])
+    assert errs[0] < 1e-4
+    assert 3.0 < errs[0] / errs[1] < 5.0
+
+
+def test_mtk_conserved_quantity_converges_with_timestep(small):
+    s = small
+    drift = []
+    for dt in (0.5, 0.125):
+        np.random.seed(4)
+        md = OracleMD(_force(s), s.pos, np.array(s.domain.rvecs), s.masses, dt * FS,
+                      baro=dict(temp=300.0, press=units.bar, timecon=1000 * FS))
+        md.propagate()
+        e1 = md.econs
+        for _ in range(int(20 / dt)):
+            md.propagate()
+        drift.append(abs(md.econs - e1))
+    assert drift[1] < 0.2 * drift[0]
+
+
+def test_mtk_pressure_equals_pressure_part(small):
+    s = small
+    P = 200.0 * units.bar
+    runs = []
+    for press, ext in ((P, 0.0), (0.0, P)):
+        np.random.seed(5)
+        md = OracleMD(_force(s), s.pos, np.array(s.domain.rvecs), s.masses, FS,
+                      thermo=dict(temp=300.0, timecon=100 * FS),
+                      baro=dict(temp=300.0, press=press, timecon=1000 * FS), ext_press=ext)
+        md.run(30)
+        runs.append(md)
+    a, b = runs
+    assert np.abs(a.pos - b.pos).max() < 1e-9
+    assert np.abs(a.rvecs - b.rvecs).max() < 1e-9
+    # econs: barostat P V term on one side equals the energy of the pressure part on the other
+    assert abs(a.econs - b.econs) < 1e-9 * abs(a.econs)
+
+
+def test_random_velocities_follow_legacy_stream():
+    m = np.linspace(1.0, 2.0, 10) * 1e7
+    np.random.seed(7)
+    v = get_random_vel(300.0, True, m)
+    np.random.seed(7)
+    z = np.random.normal(0, 1, (10, 3)) * np.sqrt(units.boltzmann * 300.0 / m).reshape(-1, 1)
+    t = np.mean(z ** 2 * m.reshape(-1, 1)) / units.boltzmann
+    assert np.array_equal(v, z * np.sqrt(300.0 / t))
+    assert abs(np.mean(v ** 2 * m.reshape(-1, 1)) / units.boltzmann - 300.0) < 1e-9
