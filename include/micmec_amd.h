@@ -74,6 +74,23 @@ int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double
  * (micmec/pes/mmff.py:135-148). */
 int mm_field_check(mm_field* f, void* stream, int32_t* flags);
 
+/* Batched objects: nsys independent systems in one device object (cell_sys [ncells] gives the
+ * system of every cell; node indices are global). No reference counterpart (the reference
+ * holds one system per ForcePartMechanical); each system's results equal those of its own
+ * mm_field_create object. */
+int mm_batch_field_create(mm_field** out, int device, int model, int32_t nsys, int64_t ncells,
+                          int64_t nnodes, const int64_t* surrounding_nodes,
+                          const int64_t* surrounding_cells, const uint8_t* wrap_bits,
+                          const int32_t* cell_row, const int32_t* cell_sys, int32_t nrows,
+                          int32_t max_states, const int32_t* nstates, const double* h0,
+                          const double* c0, const double* free_energy, const double* temp_eff,
+                          double boltzmann);
+
+/* Energies [nsys], gradient [nnodes][3] (or NULL) and virials [nsys][9] (or NULL) of a batched
+ * object at positions pos and per-system domain vectors rvecs [nsys][9]; overwrites. */
+int mm_batch_compute(mm_field* f, const double* pos, const double* rvecs, double* energies,
+                     double* gpos, double* vtens, void* stream);
+
 /* The cell pass of mm_field_compute alone (per-cell energies, gradients and virial partials
  * into the object's own buffers), for per-kernel timing and profiling; no output is written. */
 int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, void* stream);
@@ -147,6 +164,7 @@ void mm_field_destroy(mm_field* f);
 #define MM_S_VSCALE_A_ON 129
 #define MM_S_VSCALE_B 130 /* pending velocity factor applied last (thermostat pre) */
 #define MM_S_VSCALE_B_ON 131
+#define MM_S_NNODES 132 /* nodes of the system (mean of the rmsd values) */
 #define MM_S_COUNT 144
 
 typedef struct mm_md mm_md;
@@ -156,6 +174,17 @@ typedef struct mm_md mm_md;
  * step. No allocation of the caller's buffers happens here. */
 int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double* gpos,
                  const double* masses, double* posold, double* scalars, int thermo, int baro);
+
+/* Batched form: nsys independent systems integrated together. The field comes from
+ * mm_batch_field_create; node arrays hold the systems one after another, system s in rows
+ * [sys_node_start[s], sys_node_start[s + 1]) with every start a multiple of 256 (rows between
+ * a system's last node and the next start are inert padding: mass 1, zero velocity, referenced
+ * by no cell). scalars holds nsys blocks of MM_S_COUNT doubles. Every call below then advances
+ * or evaluates all systems at once, one launch per kernel. No reference counterpart: the
+ * reference integrates one system per process. */
+int mm_md_create_batch(mm_md** out, mm_field* field, int32_t nsys, const int64_t* sys_node_start,
+                       double* pos, double* vel, double* gpos, const double* masses,
+                       double* posold, double* scalars, int thermo, int baro);
 
 /* Force evaluation of VerletIntegrator.initialize (verlet.py:123-125) and of
  * MTKBarostat.init (npt.py:612-614): gpos, epot and, if with_vtens, the virial. */

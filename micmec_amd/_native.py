@@ -52,6 +52,8 @@ def _load():
     L.mm_field_info.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.mm_field_destroy.argtypes = [P]
     L.mm_field_destroy.restype = None
+    L.mm_md_destroy.argtypes = [P]
+    L.mm_md_destroy.restype = None
     for name, argtypes in _EXTRA_SIGNATURES.items():
         fn = getattr(L, name, None)
         if fn is not None:

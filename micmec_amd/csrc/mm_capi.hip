@@ -134,6 +134,9 @@ void free_field(mm_field* f) {
   (void)hipFree(f->gcell);
   (void)hipFree(f->partials);
   (void)hipFree(f->nanflag);
+  (void)hipFree(f->blk_sys);
+  (void)hipFree(f->sys_cblk_ptr);
+  (void)hipFree(f->sys_cblk_idx);
   (void)hipSetDevice(cur);
   delete f;
 }
@@ -157,12 +160,17 @@ const char* mm_version(void) { return "micmec_amd 0.1 (gfx950, fp64)"; }
 
 const char* mm_last_error(void) { return g_last_error.c_str(); }
 
-int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64_t nnodes,
-                    const int64_t* surrounding_nodes, const int64_t* surrounding_cells,
-                    const uint8_t* wrap_bits, const int32_t* cell_row, int32_t nrows,
-                    int32_t max_states, const int32_t* nstates, const double* h0,
-                    const double* c0, const double* free_energy, const double* temp_eff,
-                    double boltzmann) {
+}  // extern "C"
+
+namespace {
+
+// Shared construction of single-system and batched objects. cell_sys == nullptr means one system.
+int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t ncells,
+                 int64_t nnodes, const int64_t* surrounding_nodes,
+                 const int64_t* surrounding_cells, const uint8_t* wrap_bits,
+                 const int32_t* cell_row, const int32_t* cell_sys, int32_t nrows,
+                 int32_t max_states, const int32_t* nstates, const double* h0, const double* c0,
+                 const double* free_energy, const double* temp_eff, double boltzmann) {
   g_last_error.clear();
   if (!out) return set_error(MM_ERR_ARG, "out is NULL");
   *out = nullptr;
@@ -171,6 +179,7 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
   if (ncells < 0 || nnodes < 0) return set_error(MM_ERR_ARG, "negative sizes");
   if (ncells > (1LL << 27)) return set_error(MM_ERR_ARG, "too many cells (%lld)", (long long)ncells);
   if (nnodes > (1LL << 30)) return set_error(MM_ERR_ARG, "too many nodes");
+  if (nsys < 1 || nsys > (1 << 24)) return set_error(MM_ERR_ARG, "bad system count %d", nsys);
   if (nrows < 1 || nrows > 65535 || max_states < 1 || max_states > kMaxStates)
     return set_error(MM_ERR_ARG, "bad parameter table shape %d x %d", nrows, max_states);
   if (ncells > 0 && (!surrounding_nodes || !wrap_bits || !cell_row))
@@ -187,6 +196,8 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
     if (cell_row[c] < 0 || cell_row[c] >= nrows)
       return set_error(MM_ERR_ARG, "cell %lld has parameter row %d", (long long)c, cell_row[c]);
     if (wrap_bits[c] > 7) return set_error(MM_ERR_ARG, "wrap bits of cell %lld > 7", (long long)c);
+    if (cell_sys && (cell_sys[c] < 0 || cell_sys[c] >= nsys))
+      return set_error(MM_ERR_ARG, "cell %lld belongs to system %d", (long long)c, cell_sys[c]);
   }
   for (int64_t n = 0; n < nnodes; ++n) {
     for (int k = 0; k < 8; ++k) {
@@ -201,27 +212,41 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
                              boltzmann, tab, kt);
   if (rc) return rc;
 
-  // Work order: cells grouped by parameter row, reference order inside a row, each group
-  // padded to a whole number of blocks so that every block is type-uniform.
-  std::vector<std::vector<int64_t>> by_row(nrows);
-  for (int64_t c = 0; c < ncells; ++c) by_row[cell_row[c]].push_back(c);
+  // Work order: cells grouped by (parameter row, system), reference order inside a group, each
+  // group padded to a whole number of blocks so that every block is type- and system-uniform.
+  std::vector<std::vector<int64_t>> groups((size_t)nrows * nsys);
+  for (int64_t c = 0; c < ncells; ++c)
+    groups[(size_t)cell_row[c] * nsys + (cell_sys ? cell_sys[c] : 0)].push_back(c);
   int64_t W = 0;
-  for (int t = 0; t < nrows; ++t)
-    W += (int64_t)((by_row[t].size() + kCellBlock - 1) / kCellBlock) * kCellBlock;
-  std::vector<int32_t> cnode((size_t)8 * W, -1), cwrap(W, 0), blk_row(W / kCellBlock, 0);
+  for (const auto& g : groups) W += (int64_t)((g.size() + kCellBlock - 1) / kCellBlock) * kCellBlock;
+  std::vector<int32_t> cnode((size_t)8 * W, -1), cwrap(W, 0), blk_row(W / kCellBlock, 0),
+      blk_sys(W / kCellBlock, 0);
   std::vector<int64_t> slot(ncells, -1);
+  std::vector<std::vector<int32_t>> sys_blocks(nsys);
   int64_t w0 = 0;
   for (int t = 0; t < nrows; ++t) {
-    const int64_t nb = (int64_t)((by_row[t].size() + kCellBlock - 1) / kCellBlock);
-    for (int64_t b = 0; b < nb; ++b) blk_row[w0 / kCellBlock + b] = t;
-    for (size_t i = 0; i < by_row[t].size(); ++i) {
-      const int64_t c = by_row[t][i];
-      const int64_t w = w0 + (int64_t)i;
-      slot[c] = w;
-      for (int k = 0; k < 8; ++k) cnode[(size_t)w * 8 + k] = (int32_t)surrounding_nodes[c * 8 + k];
-      cwrap[w] = wrap_bits[c];
+    for (int sy = 0; sy < nsys; ++sy) {
+      const auto& g = groups[(size_t)t * nsys + sy];
+      const int64_t nb = (int64_t)((g.size() + kCellBlock - 1) / kCellBlock);
+      for (int64_t b = 0; b < nb; ++b) {
+        blk_row[w0 / kCellBlock + b] = t;
+        blk_sys[w0 / kCellBlock + b] = sy;
+        sys_blocks[sy].push_back((int32_t)(w0 / kCellBlock + b));
+      }
+      for (size_t i = 0; i < g.size(); ++i) {
+        const int64_t c = g[i];
+        const int64_t w = w0 + (int64_t)i;
+        slot[c] = w;
+        for (int k = 0; k < 8; ++k) cnode[(size_t)w * 8 + k] = (int32_t)surrounding_nodes[c * 8 + k];
+        cwrap[w] = wrap_bits[c];
+      }
+      w0 += nb * kCellBlock;
     }
-    w0 += nb * kCellBlock;
+  }
+  std::vector<int32_t> sys_ptr(nsys + 1, 0), sys_idx;
+  for (int sy = 0; sy < nsys; ++sy) {
+    sys_ptr[sy + 1] = sys_ptr[sy] + (int32_t)sys_blocks[sy].size();
+    sys_idx.insert(sys_idx.end(), sys_blocks[sy].begin(), sys_blocks[sy].end());
   }
   std::vector<int32_t> ncell((size_t)8 * nnodes, -1);
   for (int64_t n = 0; n < nnodes; ++n) {
@@ -240,6 +265,7 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
   f->W = W;
   f->nrows = nrows;
   f->max_states = max_states;
+  f->nsys = nsys;
   f->slot_of_cell = std::move(slot);
   const int64_t nb = W / kCellBlock;
   if ((rc = dmalloc(&f->cnode, (size_t)8 * W)) || (rc = dmalloc(&f->cwrap, W)) ||
@@ -247,7 +273,9 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
       (rc = dmalloc(&f->nstates, nrows)) || (rc = dmalloc(&f->tab, tab.size())) ||
       (rc = dmalloc(&f->kt, kt.size())) || (rc = dmalloc(&f->gcell, (size_t)24 * W)) ||
       (rc = dmalloc(&f->partials, (size_t)kPartialStride * (nb > 0 ? nb : 1))) ||
-      (rc = dmalloc(&f->nanflag, 1))) {
+      (rc = dmalloc(&f->nanflag, 1)) || (rc = dmalloc(&f->blk_sys, nb)) ||
+      (rc = dmalloc(&f->sys_cblk_ptr, (size_t)nsys + 1)) ||
+      (rc = dmalloc(&f->sys_cblk_idx, nb))) {
     free_field(f);
     return rc;
   }
@@ -255,7 +283,10 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
       (rc = h2d(f->blk_row, blk_row.data(), blk_row.size())) ||
       (rc = h2d(f->ncell, ncell.data(), ncell.size())) ||
       (rc = h2d(f->nstates, nstates, (size_t)nrows)) ||
-      (rc = h2d(f->tab, tab.data(), tab.size())) || (rc = h2d(f->kt, kt.data(), kt.size()))) {
+      (rc = h2d(f->tab, tab.data(), tab.size())) || (rc = h2d(f->kt, kt.data(), kt.size())) ||
+      (rc = h2d(f->blk_sys, blk_sys.data(), blk_sys.size())) ||
+      (rc = h2d(f->sys_cblk_ptr, sys_ptr.data(), sys_ptr.size())) ||
+      (rc = h2d(f->sys_cblk_idx, sys_idx.data(), sys_idx.size()))) {
     free_field(f);
     return rc;
   }
@@ -265,6 +296,74 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
     return hip_error(e, "hipMemset");
   }
   *out = f;
+  return MM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64_t nnodes,
+                    const int64_t* surrounding_nodes, const int64_t* surrounding_cells,
+                    const uint8_t* wrap_bits, const int32_t* cell_row, int32_t nrows,
+                    int32_t max_states, const int32_t* nstates, const double* h0,
+                    const double* c0, const double* free_energy, const double* temp_eff,
+                    double boltzmann) {
+  return create_field(out, device, model, 1, ncells, nnodes, surrounding_nodes,
+                      surrounding_cells, wrap_bits, cell_row, nullptr, nrows, max_states,
+                      nstates, h0, c0, free_energy, temp_eff, boltzmann);
+}
+
+int mm_batch_field_create(mm_field** out, int device, int model, int32_t nsys, int64_t ncells,
+                          int64_t nnodes, const int64_t* surrounding_nodes,
+                          const int64_t* surrounding_cells, const uint8_t* wrap_bits,
+                          const int32_t* cell_row, const int32_t* cell_sys, int32_t nrows,
+                          int32_t max_states, const int32_t* nstates, const double* h0,
+                          const double* c0, const double* free_energy, const double* temp_eff,
+                          double boltzmann) {
+  if (ncells > 0 && !cell_sys) return set_error(MM_ERR_ARG, "cell_sys is NULL");
+  return create_field(out, device, model, nsys, ncells, nnodes, surrounding_nodes,
+                      surrounding_cells, wrap_bits, cell_row, cell_sys, nrows, max_states,
+                      nstates, h0, c0, free_energy, temp_eff, boltzmann);
+}
+
+int mm_batch_compute(mm_field* f, const double* pos, const double* rvecs, double* energies,
+                     double* gpos, double* vtens, void* stream) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if ((f->N > 0 && !pos) || !rvecs || !energies)
+    return set_error(MM_ERR_ARG, "pos, rvecs and energies are required");
+  DeviceGuard guard(f->device);
+  ForceArgs a;
+  a.pos = pos;
+  a.rvecs = rvecs;
+  a.rvecs_stride = 9;
+  a.blk_sys = f->blk_sys;
+  a.cnode = f->cnode;
+  a.cwrap = f->cwrap;
+  a.blk_row = f->blk_row;
+  a.tab = f->tab;
+  a.kt = f->kt;
+  a.nstates = f->nstates;
+  a.max_states = f->max_states;
+  a.W = f->W;
+  a.N = f->N;
+  a.ncell = f->ncell;
+  a.gcell = f->gcell;
+  a.partials = f->partials;
+  a.energy = nullptr;
+  a.gpos = gpos;
+  a.vtens = vtens;
+  a.accumulate = 0;
+  a.model = f->model;
+  a.nanflag = f->nanflag;
+  a.force_grad = 1;
+  hipError_t e = launch_cells(a, (hipStream_t)stream);
+  if (e == hipSuccess && gpos) e = launch_gather(a, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = launch_batch_finalize(f->partials, (int)(f->W / kCellBlock), f->sys_cblk_ptr,
+                              f->sys_cblk_idx, f->nsys, energies, vtens, f->nanflag,
+                              (hipStream_t)stream);
+  if (e != hipSuccess) return hip_error(e, "batched force kernels");
   return MM_OK;
 }
 

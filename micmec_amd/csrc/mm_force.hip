@@ -77,6 +77,7 @@ __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
 template <int kModel, bool kGrad, bool kMulti>
 __global__ void __launch_bounds__(kCellBlock, 2)
 cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
+            int64_t rvecs_stride, const int32_t* __restrict__ blk_sys,
             const int32_t* __restrict__ cnode, const int32_t* __restrict__ cwrap,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
@@ -90,9 +91,10 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   const int ns = nstates_g[trow];
   const double kt = kt_g[trow];
   const double* tab = tab_g + (int64_t)trow * max_states * kTabStride;
+  const double* rvb = rvecs + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * rvecs_stride : 0);
   double rv[9];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) rv[q] = rvecs[q];
+  for (int q = 0; q < 9; ++q) rv[q] = rvb[q];
 
   const int64_t w = (int64_t)blockIdx.x * kCellBlock + threadIdx.x;
   int n[8];
@@ -210,7 +212,8 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   // launch, so the block partials are written exactly once per block.
 #define MM_LAUNCH_CELL(MODEL, GRAD, MULTI)                                                   \
   hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(nb), dim3(kCellBlock), 0,       \
-                     stream, a.pos, a.rvecs, a.cnode, a.cwrap, a.blk_row, a.tab, a.kt,       \
+                     stream, a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap,    \
+                     a.blk_row, a.tab, a.kt,                                                 \
                      a.nstates, a.max_states, a.W, a.gcell, a.partials)
   if (a.model == 0) {
     if (grad) { MM_LAUNCH_CELL(0, true, false); MM_LAUNCH_CELL(0, true, true); }
@@ -223,19 +226,67 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {
+  if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
+  const int gb = (int)((a.N + 255) / 256);
+  hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N, a.W,
+                     a.gpos, a.accumulate, a.nanflag);
+  return hipGetLastError();
+}
+
 hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
   const int nb = (int)(a.W / kCellBlock);
   hipError_t err = launch_cells(a, stream);
   if (err != hipSuccess) return err;
-  if (a.gpos != nullptr && a.N > 0 && nb > 0) {
-    const int gb = (int)((a.N + 255) / 256);
-    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                       a.W, a.gpos, a.accumulate, a.nanflag);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-  }
+  err = launch_gather(a, stream);
+  if (err != hipSuccess) return err;
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, stream, a.partials, nb, a.energy,
                      a.vtens, a.accumulate, a.nanflag);
+  return hipGetLastError();
+}
+
+// Per-system totals of a batch: one block per system sums its own cell blocks in list order.
+__global__ void __launch_bounds__(256)
+batch_finalize_kernel(const double* __restrict__ partials, int nb,
+                      const int32_t* __restrict__ sys_ptr, const int32_t* __restrict__ sys_idx,
+                      double* __restrict__ energies, double* __restrict__ vtens,
+                      int* __restrict__ nanflag) {
+  __shared__ double red[10 * 4];
+  const int s = blockIdx.x;
+  const int b0 = sys_ptr[s], b1 = sys_ptr[s + 1];
+  double acc[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int i = b0 + threadIdx.x; i < b1; i += 256) {
+    const int b = sys_idx[i];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) acc[q] += partials[(int64_t)q * nb + b];
+  }
+  wave_reduce<10>(acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) red[wave * 10 + q] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bad = 0;
+    for (int q = 0; q < 10; ++q) {
+      const double t = ((red[q] + red[10 + q]) + red[20 + q]) + red[30 + q];
+      if (t != t) bad |= q == 0 ? 1 : 4;
+      if (q == 0 && energies) energies[s] = t;
+      if (q > 0 && vtens) vtens[9 * (int64_t)s + q - 1] = t;
+    }
+    if (bad) atomicOr(nanflag, bad);
+  }
+}
+
+hipError_t launch_batch_finalize(const double* partials, int nb, const int32_t* sys_ptr,
+                                 const int32_t* sys_idx, int nsys, double* energies,
+                                 double* vtens, int* nanflag, hipStream_t stream) {
+  if (nsys <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_finalize_kernel, dim3(nsys), dim3(256), 0, stream, partials, nb,
+                     sys_ptr, sys_idx, energies, vtens, nanflag);
   return hipGetLastError();
 }
 

@@ -35,6 +35,10 @@ struct mm_field {
   double* gcell = nullptr;
   double* partials = nullptr;
   int* nanflag = nullptr;
+  int32_t nsys = 1;                   // independent systems sharing the object
+  int32_t* blk_sys = nullptr;         // [W/256] system of every cell block
+  int32_t* sys_cblk_ptr = nullptr;    // [nsys + 1] CSR of the cell blocks of every system
+  int32_t* sys_cblk_idx = nullptr;    // [W/256]
   std::vector<int64_t> slot_of_cell;  // reference cell index -> work slot
 };
 

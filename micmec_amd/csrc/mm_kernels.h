@@ -12,7 +12,9 @@ constexpr int kMaxStates = 8;  // metastable states per cell type
 
 struct ForceArgs {
   const double* pos;       // [N][3]
-  const double* rvecs;     // [3][3] device
+  const double* rvecs;     // [3][3] device (of system 0; system s at rvecs + s * rvecs_stride)
+  int64_t rvecs_stride = 0;
+  const int32_t* blk_sys = nullptr;  // [W / kCellBlock] system of each block (null: all 0)
   const int32_t* cnode;    // [W][8] corner nodes in work order (-1 = padding slot)
   const int32_t* cwrap;    // [W] wrap bits of each work slot
   const int32_t* blk_row;  // [W / kCellBlock] table row of every block
@@ -35,6 +37,12 @@ struct ForceArgs {
 };
 
 hipError_t launch_force(const ForceArgs& a, hipStream_t stream);
+// Per-system energy and virial of a batch: energies[s], vtens[9 s] (either may be null).
+hipError_t launch_batch_finalize(const double* partials, int nb, const int32_t* sys_ptr,
+                                 const int32_t* sys_idx, int nsys, double* energies,
+                                 double* vtens, int* nanflag, hipStream_t stream);
+// Node gather of gcell into gpos (overwrite or accumulate, as a.accumulate).
+hipError_t launch_gather(const ForceArgs& a, hipStream_t stream);
 // Cell kernel only: per-cell gradients into gcell, energy/virial block partials.
 hipError_t launch_cells(const ForceArgs& a, hipStream_t stream);
 

@@ -108,8 +108,10 @@ template <int kMode>
 __global__ void __launch_bounds__(kNodeBlock)
 node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ g,
             const double* __restrict__ m, const double* __restrict__ xold,
-            const double* __restrict__ S, int64_t N, double* __restrict__ partials) {
+            const double* __restrict__ S0, const int32_t* __restrict__ blk_sys, int64_t N,
+            double* __restrict__ partials) {
 #pragma clang fp contract(off)
+  const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
   const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
   double acc[13];
 #pragma unroll
@@ -167,8 +169,9 @@ template <bool kKick>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t W, double* __restrict__ gpos, double* __restrict__ v,
-                   const double* __restrict__ m, const double* __restrict__ S,
-                   double* __restrict__ partials) {
+                   const double* __restrict__ m, const double* __restrict__ S0,
+                   const int32_t* __restrict__ blk_sys, double* __restrict__ partials) {
+  const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
   const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
   double acc[13];
 #pragma unroll
@@ -486,17 +489,62 @@ __device__ void take_nodes(double* S, const double (&nd)[13]) {
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
 }
 
+// Sum SoA partials part[q * nb + b] over the blocks b of one system (a list or a range) in a
+// fixed order; every thread receives the totals.
+template <int NV>
+__device__ void reduce_blocks(const double* __restrict__ part, int nb,
+                              const int32_t* __restrict__ idx, int first, int count,
+                              double (&out)[NV]) {
+  __shared__ double red[NV * 16];
+  __shared__ double tot[NV];
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (int i = threadIdx.x; i < count; i += 1024) {
+    const int b = idx ? idx[first + i] : first + i;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += part[(int64_t)q * nb + b];
+  }
+  wave_reduce<NV>(acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave * NV + q] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double t = red[threadIdx.x];
+    for (int w = 1; w < 16; ++w) t += red[w * NV + threadIdx.x];
+    tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[q] = tot[q];
+  __syncthreads();
+}
+
+// One block per system: reduce that system's partials, then run the phase on its scalars.
 __global__ void __launch_bounds__(1024)
-scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpart, int nbf,
-              const double* __restrict__ npart, int nbn, int with_vtens, int64_t N) {
+scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpart, int nbf,
+              const int32_t* __restrict__ fsys_ptr, const int32_t* __restrict__ fsys_idx,
+              const double* __restrict__ npart, int nbn, const int32_t* __restrict__ nsys_ptr,
+              int with_vtens) {
+  const int sys = blockIdx.x;
+  double* S = S0 + (int64_t)sys * MM_S_COUNT;
   double f[10], nd[13];
   const bool use_f = phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
                      phase == kPhBaroMid || phase == kPhMainEndNpt;
   const bool use_n = phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
                      phase == kPhBaroEndPre || phase == kPhMainEndNpt ||
                      phase == kPhBaroEndPost || phase == kPhRmsd;
-  if (use_f) reduce_soa<10>(fpart, nbf, f);
-  if (use_n) reduce_soa<13>(npart, nbn, nd);
+  if (use_f) {
+    const int b0 = fsys_ptr[sys];
+    reduce_blocks<10>(fpart, nbf, fsys_idx, b0, fsys_ptr[sys + 1] - b0, f);
+  }
+  if (use_n) {
+    const int b0 = nsys_ptr[sys];
+    reduce_blocks<13>(npart, nbn, nullptr, b0, nsys_ptr[sys + 1] - b0, nd);
+  }
   if (threadIdx.x != 0) return;
   const bool thermo = S[MM_S_THERMO] != 0.0;
   switch (phase) {
@@ -588,7 +636,7 @@ scalar_kernel(int phase, double* __restrict__ S, const double* __restrict__ fpar
       properties(S, true);
       break;
     case kPhRmsd: {
-      const double denom = 3.0 * (double)N;
+      const double denom = 3.0 * S[MM_S_NNODES];
       S[MM_S_RMSD_GPOS] = sqrt(nd[10] / denom);
       S[MM_S_RMSD_DELTA] = sqrt(nd[11] / denom);
       break;
@@ -615,6 +663,9 @@ struct mm_md {
   int thermo = 0, baro = 0;
   int64_t N = 0;
   int nbn = 0;
+  int32_t nsys = 1;
+  int32_t* node_blk_sys = nullptr;   // [nbn] system of every node block
+  int32_t* sys_nblk = nullptr;       // [nsys + 1] node-block range of every system
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -639,6 +690,8 @@ ForceArgs force_args(mm_md* md) {
   ForceArgs a;
   a.pos = md->pos;
   a.rvecs = md->S + MM_S_RVECS;
+  a.rvecs_stride = MM_S_COUNT;
+  a.blk_sys = f->blk_sys;
   a.cnode = f->cnode;
   a.cwrap = f->cwrap;
   a.blk_row = f->blk_row;
@@ -665,7 +718,7 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
   const dim3 grid(md->nbn), block(kNodeBlock);
 #define MM_NODE(M)                                                                          \
   hipLaunchKernelGGL((node_kernel<M>), grid, block, 0, s, md->pos, md->vel, md->gpos,        \
-                     md->masses, md->posold, md->S, md->N, md->npart)
+                     md->masses, md->posold, md->S, md->node_blk_sys, md->N, md->npart)
   switch (mode) {
     case kKickDrift: MM_NODE(kKickDrift); break;
     case kScalePos: MM_NODE(kScalePos); break;
@@ -680,8 +733,9 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
 
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
   const int nbf = (int)(md->field->W / kCellBlock);
-  hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(1024), 0, s, phase, md->S,
-                     md->field->partials, nbf, md->npart, md->nbn, with_vtens, md->N);
+  hipLaunchKernelGGL(scalar_kernel, dim3(md->nsys), dim3(1024), 0, s, phase, md->S,
+                     md->field->partials, nbf, md->field->sys_cblk_ptr,
+                     md->field->sys_cblk_idx, md->npart, md->nbn, md->sys_nblk, with_vtens);
   return hipGetLastError();
 }
 
@@ -694,11 +748,11 @@ hipError_t force(mm_md* md, bool kick, hipStream_t s) {
     if (kick) {
       hipLaunchKernelGGL((gather_kick_kernel<true>), dim3(md->nbn), dim3(kNodeBlock), 0, s,
                          a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,
-                         md->npart);
+                         md->node_blk_sys, md->npart);
     } else {
       hipLaunchKernelGGL((gather_kick_kernel<false>), dim3(md->nbn), dim3(kNodeBlock), 0, s,
                          a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,
-                         md->npart);
+                         md->node_blk_sys, md->npart);
     }
     e = hipGetLastError();
   }
@@ -758,15 +812,32 @@ int end(mm_md* md, void* stream) {
 
 extern "C" {
 
-int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double* gpos,
-                 const double* masses, double* posold, double* scalars, int thermo, int baro) {
-  if (!out || !field || !scalars) return set_error(MM_ERR_ARG, "missing argument");
+int mm_md_create_batch(mm_md** out, mm_field* field, int32_t nsys, const int64_t* sys_node_start,
+                       double* pos, double* vel, double* gpos, const double* masses,
+                       double* posold, double* scalars, int thermo, int baro) {
+  if (!out || !field || !scalars || !sys_node_start) return set_error(MM_ERR_ARG, "missing argument");
+  if (nsys != field->nsys)
+    return set_error(MM_ERR_ARG, "field holds %d systems, not %d", field->nsys, nsys);
   if (field->N > 0 && (!pos || !vel || !gpos || !masses || !posold))
     return set_error(MM_ERR_ARG, "missing node buffer");
   if (thermo != MM_THERMO_NONE && thermo != MM_THERMO_NHC)
     return set_error(MM_ERR_ARG, "unknown thermostat %d", thermo);
   if (baro != MM_BARO_NONE && baro != MM_BARO_MTK)
     return set_error(MM_ERR_ARG, "unknown barostat %d", baro);
+  if (sys_node_start[0] != 0 || sys_node_start[nsys] != field->N)
+    return set_error(MM_ERR_ARG, "node ranges must cover [0, %lld)", (long long)field->N);
+  const int nbn_total = (int)((field->N + kNodeBlock - 1) / kNodeBlock);
+  std::vector<int32_t> node_blk_sys(nbn_total > 0 ? nbn_total : 1, 0), sys_nblk(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    const int64_t a = sys_node_start[s], b = sys_node_start[s + 1];
+    if (b < a || (nsys > 1 && a % kNodeBlock != 0))
+      return set_error(MM_ERR_ARG, "system %d: node range [%lld, %lld) invalid", s,
+                       (long long)a, (long long)b);
+    const int blo = (int)(a / kNodeBlock), bhi = (int)((b + kNodeBlock - 1) / kNodeBlock);
+    sys_nblk[s] = blo;
+    sys_nblk[s + 1] = bhi;
+    for (int k = blo; k < bhi; ++k) node_blk_sys[k] = s;
+  }
   Dev dev(field->device);
   mm_md* md = new mm_md();
   md->field = field;
@@ -779,9 +850,17 @@ int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double*
   md->thermo = thermo;
   md->baro = baro;
   md->N = field->N;
-  md->nbn = (int)((field->N + kNodeBlock - 1) / kNodeBlock);
-  if (md->nbn == 0) md->nbn = 1;
+  md->nbn = nbn_total > 0 ? nbn_total : 1;
+  md->nsys = nsys;
   hipError_t e = hipMalloc((void**)&md->npart, sizeof(double) * kNodePartial * md->nbn);
+  if (e == hipSuccess) e = hipMalloc((void**)&md->node_blk_sys, sizeof(int32_t) * md->nbn);
+  if (e == hipSuccess) e = hipMalloc((void**)&md->sys_nblk, sizeof(int32_t) * (nsys + 1));
+  if (e == hipSuccess)
+    e = hipMemcpy(md->node_blk_sys, node_blk_sys.data(), sizeof(int32_t) * md->nbn,
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(md->sys_nblk, sys_nblk.data(), sizeof(int32_t) * (nsys + 1),
+                  hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&md->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&md->ev_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&md->ev_out, hipEventDisableTiming);
@@ -791,6 +870,15 @@ int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double*
   }
   *out = md;
   return MM_OK;
+}
+
+int mm_md_create(mm_md** out, mm_field* field, double* pos, double* vel, double* gpos,
+                 const double* masses, double* posold, double* scalars, int thermo, int baro) {
+  if (!field) return set_error(MM_ERR_ARG, "missing argument");
+  if (field->nsys != 1) return set_error(MM_ERR_ARG, "batched field: use mm_md_create_batch");
+  const int64_t range[2] = {0, field->N};
+  return mm_md_create_batch(out, field, 1, range, pos, vel, gpos, masses, posold, scalars,
+                            thermo, baro);
 }
 
 int mm_md_init_force(mm_md* md, int with_vtens, void* stream) {
@@ -886,6 +974,8 @@ void mm_md_destroy(mm_md* md) {
   if (md->ev_in) (void)hipEventDestroy(md->ev_in);
   if (md->ev_out) (void)hipEventDestroy(md->ev_out);
   if (md->npart) (void)hipFree(md->npart);
+  if (md->node_blk_sys) (void)hipFree(md->node_blk_sys);
+  if (md->sys_nblk) (void)hipFree(md->sys_nblk);
   delete md;
 }
 

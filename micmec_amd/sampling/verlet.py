@@ -370,6 +370,7 @@ class VerletIntegrator(Iterative):
         S[L.S_EXT_PRESS] = self._engine.ext_press
         S[L.S_TIME] = self._time0
         S[L.S_NDOF] = 1.0
+        S[L.S_NNODES] = self.mmf.system.nnodes
         self._engine.put_scalars(S)
         self.mmf.update_pos(self._pos)
         self._init_force(with_vtens=False)
