@@ -45,6 +45,11 @@ def parse():
     ap.add_argument("--cpu-grid", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel-launches", type=int, default=20)
+    ap.add_argument("--batched", dest="batched", action="store_true", default=True)
+    ap.add_argument("--no-batched", dest="batched", action="store_false")
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--sequential-steps", type=int, default=20)
     return ap.parse_args()
 
 
@@ -148,6 +153,54 @@ def cpu_baseline(n, seconds, model):
                       "same workload, %d NPT steps in %.1f s" % (n, steps, dt)}
 
 
+def batched_vs_sequential(count, steps, seq_steps):
+    """Aggregate cell-steps/s of the replica set run as one batch and one replica at a time."""
+    import gc
+    import numpy as np
+    import torch
+    from micmec_amd import units
+    from micmec_amd.batch import BatchVerlet
+    from micmec_amd.pes import ForcePartMechanical, MicMecForceField
+    from micmec_amd.sampling import NHCThermostat, VerletIntegrator
+    from micmec_amd.workloads import batch_replicas
+    reps = batch_replicas(count)
+    cells = np.array([int(np.prod(np.shape(s.grid))) for s, _, _ in reps])
+    fs = units.femtosecond
+    batch = BatchVerlet([r[0] for r in reps], 1.0 * fs, [r[1] for r in reps],
+                        [r[2] for r in reps], thermostat=True)
+    batch.run(5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    del batch
+    gc.collect()
+    t_seq = 0.0
+    for s, temp, seed in reps:
+        mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+        np.random.seed(seed)
+        v = VerletIntegrator(mmf, timestep=1.0 * fs, temp0=temp,
+                             hooks=[NHCThermostat(temp=temp, timecon=100 * fs)])
+        v.run(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        v.run(seq_steps)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+        del v, mmf
+    value_b = cells.sum() * steps / t_batch
+    value_s = cells.sum() * seq_steps / t_seq
+    return {"workload": "batch%d" % count, "replicas": count,
+            "grid_cells": int(cells.sum()), "ensemble": "NVT NHC(3), dt 1 fs",
+            "batched": {"value": value_b, "unit": "cell-steps/s", "steps": steps,
+                        "ms_per_step": t_batch / steps * 1e3},
+            "one_at_a_time": {"value": value_s, "unit": "cell-steps/s",
+                              "steps_per_replica": seq_steps,
+                              "ms_per_step_all_replicas": t_seq / seq_steps * 1e3},
+            "speedup": value_b / value_s}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -203,6 +256,10 @@ def main():
                    "parallelism": "single GPU", "graph": not args.no_graph},
         "roofline": roofline,
     }
+    if args.batched:
+        print("batched replicas ...", file=sys.stderr, flush=True)
+        out["batched_replicas"] = batched_vs_sequential(args.replicas, args.batched_steps,
+                                                        args.sequential_steps)
     if args.cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_grid, args.cpu_seconds, args.model)
     print(json.dumps(out))

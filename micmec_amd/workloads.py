@@ -19,7 +19,7 @@ from .system import System
 from .topology import build_system, build_type
 
 __all__ = ["cubic_elasticity", "type_a", "type_b", "make_grid", "make_system", "workload",
-           "WORKLOADS"]
+           "batch_replicas", "WORKLOADS"]
 
 _V = {0: (0, 0), 1: (1, 1), 2: (2, 2), 3: (1, 2), 4: (0, 2), 5: (0, 1)}
 
@@ -99,6 +99,23 @@ def make_system(shape, p_b=0.0, p_hole=0.0, seed=0, noise=0.05 * units.angstrom,
     out["pos"] = pos
     out["rvecs"] = rvecs
     return System.from_dict(out)
+
+
+def batch_replicas(count=512, seed=4000, nmin=4, nmax=24):
+    """Replica set of the batched workload: grid edge n uniform in {nmin..nmax}, a seeded type
+    mix (probability of type B uniform in [0, 1]), a defect fraction uniform in [0, 0.3] and an
+    initial temperature uniform in [100, 500] K. Returns (system, temperature, seed) triples;
+    replica r uses seed + r for its grid and its velocities."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        p_b = float(rng.uniform(0.0, 1.0))
+        p_hole = float(rng.uniform(0.0, 0.3))
+        temp = float(rng.uniform(100.0, 500.0))
+        out.append((make_system((n, n, n), p_b=p_b, p_hole=p_hole, seed=seed + r), temp,
+                    seed + r))
+    return out
 
 
 def workload(name, **overrides):
