@@ -100,6 +100,49 @@ int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* 
 
 void mm_field_destroy(mm_field* f);
 
+/* Replace the cell-type parameters of an existing object (same number of rows and at most
+ * max_states states per row), with the elasticity given as 6x6 Voigt matrices:
+ * cvoigt [nrows][max_states][6][6] with u = e.Cv.e / 2 for the engineering strain
+ * e = (eps00, eps11, eps22, 2 eps12, 2 eps02, 2 eps01); for a tensor C0 of full symmetry
+ * Cv[I][J] = C0[i][j][k][l] with (ij) = I and (kl) = J in that order (micmec/analysis/tensor.py
+ * voigt map). Other arguments as in mm_field_create. Synchronises `stream` first, then copies
+ * the new table; later work on any stream sees it. No reference counterpart: the reference
+ * reads parameters once (micmec/pes/mmff.py:218-231); this is the entry point of parameter
+ * fitting. */
+int mm_field_set_params(mm_field* f, const int32_t* nstates, const double* h0,
+                        const double* cvoigt, const double* free_energy, const double* temp_eff,
+                        double boltzmann, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Differentiation form. No reference counterpart for the parameter derivatives; the position
+ * and deformation derivatives agree with ForcePartMechanical (micmec/pes/mmff.py:288-323) and
+ * with the automatic derivatives of micmec/pes/mmff_jax.py:249-250.
+ *
+ * mm_adjoint_create allocates the work buffers of an object (about (2 * npar + 38) / 32 + 24
+ * doubles per work slot) and returns npar = max_states * 32 + 1, the parameter entries per row.
+ *
+ * mm_adjoint_compute evaluates, at positions pos [nnodes][3] and domain vectors rvecs [nsys][9]:
+ *   sys_out [nsys][38]: energy, virial [9], dE/d(domain vectors) [9] (row-major, vector d at
+ *     3d..3d+2), then the tangents of those 19 numbers;
+ *   gpos [nnodes][3] (or NULL) and, with a direction, its tangent dgpos (or NULL);
+ *   params [nrows][npar][2] (value, tangent): per state s of the row, entries s*32 + q:
+ *     q = 0..20  dE/dCv: 1/2 sum e_I e_J, (I <= J) row by row of the upper triangle; the
+ *                gradient of the 6x6 matrix has this value at (I,J) and at (J,I)
+ *     q = 21..29 dE/dP, P = h0^-1 row-major;   q = 30 dE/dw, w = det(h0)/8 (corner8) or
+ *                det(h0) (original);          q = 31 dE/d(free energy)
+ *   and entry max_states*32: dE/d(kT), kT = boltzmann * temp_eff.
+ * The tangent direction (dpos [nnodes][3], drvecs [nsys][9], both or neither) turns on dual
+ * numbers: every tangent is the derivative of its value along (dpos, drvecs). energy_weight
+ * [nsys] (or NULL = all 1) scales each system's share of the parameter values (not of their
+ * tangents). All pointers are device pointers; outputs are overwritten. The vertex gradients
+ * are recomputed from the positions on every call: nothing is stored between calls. */
+typedef struct mm_adjoint mm_adjoint;
+int mm_adjoint_create(mm_adjoint** out, mm_field* field, int64_t* npar);
+int mm_adjoint_compute(mm_adjoint* a, const double* pos, const double* dpos, const double* rvecs,
+                       const double* drvecs, const double* energy_weight, double* sys_out,
+                       double* gpos, double* dgpos, double* params, void* stream);
+void mm_adjoint_destroy(mm_adjoint* a);
+
 /* ------------------------------------------------------------------------------------------
  * Device MD engine: velocity Verlet with an optional Nose-Hoover chain and MTK barostat,
  * replacing VerletIntegrator.propagate (micmec/sampling/verlet.py:140-190), NHChain /

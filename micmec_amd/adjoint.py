@@ -1,0 +1,414 @@
+"""Differentiation form: energy, forces and virial as differentiable functions of node
+positions, domain vectors and the cell-type parameters.
+
+``micmec_energy(field, pos, rvecs, params)`` returns per-system energies [nsys], the node
+gradient gpos [N, 3] and the virials [nsys, 3, 3] of a :class:`DiffField` (one or many systems
+in one device object) and is a ``torch.autograd.Function``: a loss built from any of the three
+outputs back-propagates to ``pos``, ``rvecs`` and the parameter tensors
+
+* ``cv``   [T, S, 6, 6]  symmetric Voigt elasticity matrix of every (type, state),
+  u = e.Cv.e / 2 with the engineering strain e = (eps00, eps11, eps22, 2 eps12, 2 eps02,
+  2 eps01); its gradient is returned symmetric and holds for symmetric perturbations (the 21
+  independent entries). The forces use the stress Cv e as the reference does (sigma = C:eps,
+  micmec/pes/nanocell.py:98-132), which is the gradient of the energy only for symmetric Cv;
+* ``h0``   [T, S, 3, 3]  rest cell matrix (rows = cell vectors);
+* ``free`` [T, S]        free energy of every state;
+* ``teff`` [T]           effective temperature (kelvin) of the state combination.
+
+The energy model is the reference's (micmec/pes/nanocell.py:40-132, nanocell_original.py:46-132,
+micmec/pes/mmff.py:380-398); the reference itself differentiates only with respect to positions
+(mmff.py:384-398) and, in mmff_jax.py:249-250, a deformation of the domain. The forward pass is
+the plain force path (``mm_batch_compute``); the backward pass is one launch of the hand-written
+adjoint kernel in dual numbers (``mm_adjoint_compute``), which recomputes everything from the
+positions: no per-corner intermediate is stored between the passes.
+
+Backward, for cotangents (gE, gG, gV) of (E, gpos, vtens): with the tangent direction
+u = (dpos, drvecs) = (gG + pos gV^T, rvecs gV^T) the kernel returns the values (gpos, dE/drvecs,
+dE/dtheta) and their derivatives along u; then
+
+    dL/dpos   = gE gpos + d(gpos)[u] + gpos gV
+    dL/drvecs = gE dE/drvecs + d(dE/drvecs)[u] + (dE/drvecs) gV
+    dL/dtheta = gE dE/dtheta + d(dE/dtheta)[u]
+
+(the last terms of the first two lines come from vtens = sum_v g_v (x) r_v, whose derivative
+along a deformation D is the derivative of E along (pos D, rvecs D)). The parameter entries of
+the kernel (derivatives with respect to Cv, P = h0^-1, the weight w = det(h0)/8 or det(h0), the
+free energies and kT) are mapped here to (cv, h0, free, teff).
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+from torch.autograd.function import once_differentiable
+
+from . import _native, units
+from .batch import concat_systems
+
+__all__ = ["DiffField", "CellParams", "micmec_energy", "voigt_from_tensor", "tensor_from_voigt",
+           "voigt_pairs", "fit_workload", "FitProblem", "fit"]
+
+_P = ctypes.c_void_p
+_i32, _i64, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+_native.register_signature("mm_batch_field_create",
+                           [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_int, _i32, _i64, _i64, _P,
+                            _P, _P, _P, _P, _i32, _i32, _P, _P, _P, _P, _P, _f64])
+_native.register_signature("mm_batch_compute", [_P, _P, _P, _P, _P, _P, _P])
+_native.register_signature("mm_field_set_params", [_P, _P, _P, _P, _P, _P, _f64, _P])
+_native.register_signature("mm_adjoint_create", [ctypes.POINTER(_P), _P, ctypes.POINTER(_i64)])
+_native.register_signature("mm_adjoint_compute", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+
+NPS = 32       # parameter entries per (type, state) of the adjoint kernel
+SYSQ = 19      # energy, virial (9), domain-vector gradient (9)
+# Voigt index -> tensor index pair (micmec/analysis/tensor.py:188-197)
+_VOIGT = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))
+
+
+def voigt_pairs():
+    """The 21 (I, J), I <= J, entries of the upper triangle, row by row."""
+    return [(I, J) for I in range(6) for J in range(I, 6)]
+
+
+def voigt_from_tensor(C):
+    """6x6 Voigt matrix of a 3x3x3x3 elasticity tensor of full symmetry (leading axes kept)."""
+    C = np.asarray(C, dtype=np.float64)
+    out = np.zeros(C.shape[:-4] + (6, 6))
+    for I, (i, j) in enumerate(_VOIGT):
+        for J, (k, l) in enumerate(_VOIGT):
+            out[..., I, J] = C[..., i, j, k, l]
+    return out
+
+
+def tensor_from_voigt(cv):
+    """3x3x3x3 tensor with C[i,j,k,l] = cv[I(ij), J(kl)] (leading axes kept)."""
+    cv = np.asarray(cv, dtype=np.float64)
+    inv = {}
+    for I, (i, j) in enumerate(_VOIGT):
+        inv[(i, j)] = inv[(j, i)] = I
+    out = np.zeros(cv.shape[:-2] + (3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    out[..., i, j, k, l] = cv[..., inv[(i, j)], inv[(k, l)]]
+    return out
+
+
+class CellParams(object):
+    """Differentiable cell-type parameters (float64 tensors) and the fixed state counts."""
+
+    def __init__(self, cv, h0, free, teff, nstates, type_ids=None):
+        self.cv, self.h0, self.free, self.teff = cv, h0, free, teff
+        self.nstates = np.asarray(nstates, dtype=np.int32)
+        self.type_ids = list(type_ids) if type_ids is not None else list(range(len(nstates)))
+
+    @classmethod
+    def from_table(cls, table, device, requires_grad=False):
+        ids, ns, h0, c0, free, temp = table
+        kw = dict(dtype=torch.float64, device=device)
+        t = [torch.tensor(voigt_from_tensor(c0), **kw), torch.tensor(h0, **kw),
+             torch.tensor(free, **kw), torch.tensor(temp, **kw)]
+        if requires_grad:
+            for x in t:
+                x.requires_grad_(True)
+        return cls(*t, ns, ids)
+
+    def tensors(self):
+        return self.cv, self.h0, self.free, self.teff
+
+    def detached(self):
+        return CellParams(*[x.detach().clone() for x in self.tensors()], self.nstates,
+                          self.type_ids)
+
+
+class DiffField(object):
+    """One or more systems in one batched device object, with the adjoint work buffers.
+
+    Nodes of system k occupy rows ``node_start[k]:node_start[k] + nnodes_k`` of the position
+    array; ``node_sys`` [N] gives each row's system (padding rows: the preceding system, and
+    they carry no cell)."""
+
+    def __init__(self, systems, model="corner8", device=None):
+        if model not in _native.MODELS:
+            raise ValueError("Unknown model %r; choose one of %s" % (model, list(_native.MODELS)))
+        self.systems = list(systems)
+        self.nsys = len(self.systems)
+        if self.nsys < 1:
+            raise ValueError("need at least one system")
+        self.model = model
+        self.device = torch.device(device if device is not None else "cuda")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        topo = concat_systems(self.systems, 1)
+        self.table = topo["types"]
+        ids, ns, h0, c0, free, temp = self.table
+        self.nrows, self.max_states = len(ids), int(h0.shape[1])
+        self.node_start = topo["node_start"]
+        self.N = topo["nnodes"]
+        self.ncells = topo["ncells"]
+        self.lib = _native.lib()
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        handle = ctypes.c_void_p()
+        rc = self.lib.mm_batch_field_create(
+            ctypes.byref(handle), self.device.index, _native.MODELS[model], self.nsys,
+            topo["ncells"], self.N, vp(topo["surrounding_nodes"]), vp(topo["surrounding_cells"]),
+            vp(topo["wrap_bits"]), vp(topo["cell_row"]), vp(topo["cell_sys"]), self.nrows,
+            self.max_states, vp(ns), vp(h0), vp(c0), vp(free), vp(temp),
+            float(units.boltzmann))
+        _native.check(rc, "mm_batch_field_create")
+        self._field = handle
+        adj = ctypes.c_void_p()
+        npar = _i64(0)
+        rc = self.lib.mm_adjoint_create(ctypes.byref(adj), self._field, ctypes.byref(npar))
+        if rc != _native.MM_OK:
+            self.lib.mm_field_destroy(self._field)
+            self._field = None
+        _native.check(rc, "mm_adjoint_create")
+        self._adj = adj
+        self.npar = int(npar.value)
+        counts = np.diff(self.node_start)
+        self.node_sys = torch.tensor(np.repeat(np.arange(self.nsys), counts), dtype=torch.long,
+                                     device=self.device)
+
+    # ------------------------------------------------------------------ inputs
+    def positions(self):
+        """Concatenated positions [N, 3] and domain vectors [nsys, 3, 3] of the systems."""
+        pos = np.zeros((self.N, 3))
+        rv = np.zeros((self.nsys, 3, 3))
+        for k, s in enumerate(self.systems):
+            a = self.node_start[k]
+            pos[a:a + s.nnodes] = s.pos
+            r = np.asarray(s.domain.rvecs)
+            rv[k, : len(r)] = r
+        kw = dict(dtype=torch.float64, device=self.device)
+        return torch.tensor(pos, **kw), torch.tensor(rv, **kw)
+
+    def initial_params(self, requires_grad=False):
+        return CellParams.from_table(self.table, self.device, requires_grad)
+
+    def set_params(self, params):
+        """Upload ``params`` (a CellParams) as the object's parameter table (synchronises the
+        current stream; the table is small)."""
+        hp = [np.ascontiguousarray(x.detach().cpu().numpy(), dtype=np.float64)
+              for x in params.tensors()]
+        ns = np.ascontiguousarray(params.nstates, dtype=np.int32)
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        rc = self.lib.mm_field_set_params(self._field, vp(ns), vp(hp[1]), vp(hp[0]), vp(hp[2]),
+                                          vp(hp[3]), float(units.boltzmann),
+                                          _native.stream_ptr())
+        _native.check(rc, "mm_field_set_params")
+
+    # ------------------------------------------------------------------ kernels
+    def compute(self, pos, rvecs):
+        """Energies [nsys], gpos [N, 3], vtens [nsys, 3, 3] (plain force path)."""
+        kw = dict(dtype=torch.float64, device=self.device)
+        e = torch.empty(self.nsys, **kw)
+        g = torch.empty((self.N, 3), **kw)
+        v = torch.empty((self.nsys, 3, 3), **kw)
+        rc = self.lib.mm_batch_compute(self._field, _native.ptr(pos), _native.ptr(rvecs),
+                                       _native.ptr(e), _native.ptr(g), _native.ptr(v),
+                                       _native.stream_ptr())
+        _native.check(rc, "mm_batch_compute")
+        return e, g, v
+
+    def adjoint(self, pos, rvecs, dpos=None, drvecs=None, energy_weight=None):
+        """One adjoint launch. Returns a dict of device tensors: ``energy`` [nsys],
+        ``vtens`` [nsys, 3, 3], ``grvecs`` [nsys, 3, 3], ``gpos`` [N, 3], ``params``
+        [nrows, npar] (raw kernel entries) and, with a direction, the tangents ``d*`` of each."""
+        kw = dict(dtype=torch.float64, device=self.device)
+        dual = dpos is not None
+        if dual != (drvecs is not None):
+            raise ValueError("dpos and drvecs go together")
+        for t in (pos, rvecs, dpos, drvecs, energy_weight):
+            if t is not None and (t.dtype != torch.float64 or not t.is_contiguous()
+                                  or t.device != self.device):
+                raise TypeError("adjoint inputs must be contiguous float64 tensors on %s"
+                                % self.device)
+        if pos.shape != (self.N, 3) or rvecs.numel() != 9 * self.nsys:
+            raise ValueError("pos must be [%d, 3] and rvecs [%d, 3, 3]" % (self.N, self.nsys))
+        if dual and (dpos.shape != pos.shape or drvecs.numel() != rvecs.numel()):
+            raise ValueError("direction shapes differ from pos / rvecs")
+        if energy_weight is not None and energy_weight.numel() != self.nsys:
+            raise ValueError("energy_weight must have one entry per system")
+        sys_out = torch.empty((self.nsys, 2 * SYSQ), **kw)
+        gpos = torch.empty((self.N, 3), **kw)
+        dgpos = torch.empty((self.N, 3), **kw) if dual else None
+        params = torch.empty((self.nrows, self.npar, 2), **kw)
+        rc = self.lib.mm_adjoint_compute(self._adj, _native.ptr(pos), _native.ptr(dpos),
+                                         _native.ptr(rvecs), _native.ptr(drvecs),
+                                         _native.ptr(energy_weight), _native.ptr(sys_out),
+                                         _native.ptr(gpos), _native.ptr(dgpos),
+                                         _native.ptr(params), _native.stream_ptr())
+        _native.check(rc, "mm_adjoint_compute")
+        out = dict(energy=sys_out[:, 0], vtens=sys_out[:, 1:10].reshape(-1, 3, 3),
+                   grvecs=sys_out[:, 10:19].reshape(-1, 3, 3), gpos=gpos,
+                   params=params[:, :, 0])
+        if dual:
+            out.update(denergy=sys_out[:, SYSQ], dvtens=sys_out[:, SYSQ + 1:SYSQ + 10]
+                       .reshape(-1, 3, 3), dgrvecs=sys_out[:, SYSQ + 10:].reshape(-1, 3, 3),
+                       dgpos=dgpos, dparams=params[:, :, 1])
+        return out
+
+    def param_grads(self, entries, params):
+        """Map raw kernel entries [nrows, npar] to gradients of (cv, h0, free, teff)."""
+        cv, h0, free, teff = [x.detach() for x in params.tensors()]
+        T, S = self.nrows, self.max_states
+        ent = entries[:, : S * NPS].reshape(T, S, NPS)
+        g_cv = torch.zeros((T, S, 6, 6), dtype=torch.float64, device=self.device)
+        for q, (I, J) in enumerate(voigt_pairs()):
+            g_cv[:, :, I, J] = ent[:, :, q]
+            g_cv[:, :, J, I] = ent[:, :, q]
+        gP = ent[:, :, 21:30].reshape(T, S, 3, 3)
+        gw = ent[:, :, 30]
+        h0n = h0.cpu().numpy()
+        kw = dict(dtype=torch.float64, device=self.device)
+        Pt = torch.tensor(np.swapaxes(np.linalg.inv(h0n), -1, -2), **kw)
+        det = torch.tensor(np.linalg.det(h0n), **kw)
+        w = det / 8.0 if self.model == "corner8" else det
+        g_h0 = -(Pt @ gP @ Pt) + (gw * w)[..., None, None] * Pt
+        g_free = ent[:, :, 31].clone()
+        g_teff = entries[:, S * NPS] * units.boltzmann
+        mask = torch.tensor(np.arange(S)[None, :] < params.nstates[:, None], device=self.device)
+        g_cv = g_cv * mask[..., None, None]
+        g_h0 = g_h0 * mask[..., None, None]
+        g_free = g_free * mask
+        return g_cv, g_h0, g_free, g_teff
+
+    def __del__(self):
+        lib = getattr(self, "lib", None)
+        if lib is None:
+            return
+        a = getattr(self, "_adj", None)
+        if a is not None and a.value:
+            lib.mm_adjoint_destroy(a)
+            self._adj = None
+        f = getattr(self, "_field", None)
+        if f is not None and f.value:
+            lib.mm_field_destroy(f)
+            self._field = None
+
+
+_native.register_signature("mm_adjoint_destroy", [_P])
+
+
+class _MicMecEnergy(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, field, params, pos, rvecs, cv, h0, free, teff):
+        field.set_params(params)
+        pos_c = pos.detach().contiguous()
+        rv_c = rvecs.detach().reshape(field.nsys, 3, 3).contiguous()
+        e, g, v = field.compute(pos_c, rv_c)
+        ctx.field = field
+        ctx.params = params
+        ctx.rv_shape = rvecs.shape
+        ctx.save_for_backward(pos_c, rv_c)
+        return e, g, v
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, gE, gG, gV):
+        field, params = ctx.field, ctx.params
+        pos, rv = ctx.saved_tensors
+        kw = dict(dtype=torch.float64, device=pos.device)
+        gE = torch.zeros(field.nsys, **kw) if gE is None else gE.contiguous()
+        gG = torch.zeros_like(pos) if gG is None else gG.contiguous()
+        gV = (torch.zeros((field.nsys, 3, 3), **kw) if gV is None
+              else gV.contiguous().reshape(-1, 3, 3))
+        field.set_params(params)
+        gV_node = gV[field.node_sys]                                   # [N, 3, 3]
+        gVt = gV.transpose(1, 2).contiguous()
+        dpos = (gG + torch.matmul(pos.unsqueeze(1), gV_node.transpose(1, 2).contiguous())
+                .squeeze(1)).contiguous()
+        drv = torch.matmul(rv, gVt).contiguous()
+        out = field.adjoint(pos, rv, dpos, drv, energy_weight=gE)
+        gpos, grv = out["gpos"], out["grvecs"]
+        g_pos = (gE[field.node_sys, None] * gpos + out["dgpos"]
+                 + torch.matmul(gpos.unsqueeze(1), gV_node).squeeze(1))
+        g_rv = gE[:, None, None] * grv + out["dgrvecs"] + torch.matmul(grv, gV)
+        g_cv, g_h0, g_free, g_teff = field.param_grads(out["params"] + out["dparams"], params)
+        return (None, None, g_pos, g_rv.reshape(ctx.rv_shape), g_cv, g_h0, g_free, g_teff)
+
+
+def micmec_energy(field, pos, rvecs, params):
+    """Differentiable (energies [nsys], gpos [N, 3], vtens [nsys, 3, 3]) of ``field`` at
+    ``pos`` [N, 3] and ``rvecs`` [nsys, 3, 3] with parameters ``params`` (a CellParams)."""
+    return _MicMecEnergy.apply(field, params, pos, rvecs, *params.tensors())
+
+
+# ---------------------------------------------------------------------------- fit workload
+class FitProblem(object):
+    """Force-and-stress matching of cell-type parameters (the differentiation workload).
+
+    Targets: forces -gpos and stresses -vtens / V of every configuration, computed with the
+    target parameters. Loss: mean squared force error over nodes plus mean squared stress error
+    over configurations, each divided by the squared scale of its targets."""
+
+    def __init__(self, field, pos, rvecs, target_params):
+        self.field, self.pos, self.rvecs = field, pos, rvecs
+        with torch.no_grad():
+            e, g, v = micmec_energy(field, pos, rvecs, target_params)
+        self.volume = torch.tensor(np.abs(np.linalg.det(rvecs.cpu().numpy())),
+                                   dtype=torch.float64, device=rvecs.device)
+        self.f_target = -g
+        self.s_target = -v / self.volume[:, None, None]
+        self.f_scale = float((self.f_target ** 2).mean())
+        self.s_scale = float((self.s_target ** 2).mean())
+
+    def loss(self, params):
+        e, g, v = micmec_energy(self.field, self.pos, self.rvecs, params)
+        lf = ((-g - self.f_target) ** 2).mean() / self.f_scale
+        ls = ((-v / self.volume[:, None, None] - self.s_target) ** 2).mean() / self.s_scale
+        return lf + ls
+
+
+def fit_workload(count=64, n=16, seed=3, c_noise=0.10, device=None, model="corner8"):
+    """The differentiation workload: ``count`` configurations on an n^3 grid with types A and B
+    (probability of B 0.2), node noise 0.1 Angstrom and a +-3% random symmetric strain
+    (configuration i seeded with seed + i); targets from the elasticity matrices of every
+    (type, state) perturbed by ``c_noise`` relative seeded noise (symmetric). Returns
+    (FitProblem, initial CellParams with gradients on cv and h0, target CellParams)."""
+    from .workloads import make_system
+    systems = [make_system((n, n, n), p_b=0.2, seed=seed + i, noise=0.1 * units.angstrom,
+                           strain=0.03) for i in range(count)]
+    field = DiffField(systems, model=model, device=device)
+    pos, rvecs = field.positions()
+    target = field.initial_params()
+    rng = np.random.default_rng(seed + 1000)
+    noise = rng.uniform(-c_noise, c_noise, tuple(target.cv.shape))
+    noise = 0.5 * (noise + np.swapaxes(noise, -1, -2))
+    target.cv = target.cv * (1.0 + torch.tensor(noise, dtype=torch.float64,
+                                                device=target.cv.device))
+    problem = FitProblem(field, pos, rvecs, target)
+    init = field.initial_params()
+    init.cv.requires_grad_(True)
+    init.h0.requires_grad_(True)
+    return problem, init, target
+
+
+def fit(problem, init, steps, lr_cv=1e-2, lr_h0=1e-4, callback=None):
+    """Adam on relative corrections of the elasticity matrices and rest cells:
+    cv = cv_init * (1 + a) with a symmetric, h0 = h0_init * (1 + b). Returns the loss history
+    and the final CellParams."""
+    cv0, h00 = init.cv.detach(), init.h0.detach()
+    free, teff = init.free.detach(), init.teff.detach()
+    a = torch.zeros_like(cv0, requires_grad=True)
+    b = torch.zeros_like(h00, requires_grad=True)
+    opt = torch.optim.Adam([{"params": [a], "lr": lr_cv}, {"params": [b], "lr": lr_h0}])
+
+    def params_of(a, b):
+        asym = 0.5 * (a + a.transpose(-1, -2))
+        return CellParams(cv0 * (1.0 + asym), h00 * (1.0 + b), free, teff, init.nstates,
+                          init.type_ids)
+
+    losses = []
+    for it in range(steps):
+        opt.zero_grad()
+        loss = problem.loss(params_of(a, b))
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+        if callback is not None:
+            callback(it, losses[-1])
+    with torch.no_grad():
+        final = params_of(a, b)
+    return losses, final

@@ -24,7 +24,7 @@ from . import _native, units
 from .pes.mmff import parse_type_params, system_wrap_bits
 from .sampling.utils import clean_momenta, get_ndof_internal_md, get_random_vel
 
-__all__ = ["BatchVerlet", "NODE_ALIGN"]
+__all__ = ["BatchVerlet", "NODE_ALIGN", "concat_systems"]
 
 NODE_ALIGN = 256   # node block of the device engine; each system starts on a block boundary
 
@@ -68,6 +68,47 @@ def _type_table(systems):
     return ids, ns, h0, c0, free, temp
 
 
+def concat_systems(systems, node_align=1):
+    """Concatenated topology of several systems for one batched device object.
+
+    System k's nodes start at ``node_start[k]`` (a multiple of ``node_align``; the gap up to the
+    next system holds padding nodes without cells), its cells follow those of system k - 1.
+    Returns a dict with ``surrounding_nodes``, ``surrounding_cells``, ``wrap_bits``,
+    ``cell_row``, ``cell_sys``, ``node_start`` [nsys + 1], ``ncells``, ``nnodes`` and the shared
+    parameter table ``types`` = (ids, nstates, h0, c0, free, temp) of :func:`_type_table`."""
+    ids, ns, h0, c0, free, temp = _type_table(systems)
+    row_of = {t: r for r, t in enumerate(ids)}
+    starts = [0]
+    for s in systems:
+        starts.append(starts[-1] + -(-s.nnodes // node_align) * node_align)
+    sn_all, sc_all, wb_all, row_all, sys_all = [], [], [], [], []
+    cell_off = 0
+    for k, s in enumerate(systems):
+        sn = np.asarray(s.surrounding_nodes, dtype=np.int64) + starts[k]
+        sc = np.asarray(s.surrounding_cells, dtype=np.int64)
+        sc = np.where(sc >= 0, sc + cell_off, -1)
+        pad = np.full((starts[k + 1] - starts[k] - s.nnodes, 8), -1, dtype=np.int64)
+        sn_all.append(sn)
+        sc_all.extend([sc, pad])
+        wb_all.append(system_wrap_bits(s))
+        types = np.rint(np.asarray(s.types, dtype=float)).astype(int)
+        row_all.append(np.array([row_of[int(t)] for t in types], dtype=np.int32))
+        sys_all.append(np.full(s.ncells, k, dtype=np.int32))
+        cell_off += s.ncells
+    sn = np.ascontiguousarray(np.concatenate(sn_all))
+    return dict(
+        surrounding_nodes=sn,
+        surrounding_cells=np.ascontiguousarray(np.concatenate(sc_all)),
+        wrap_bits=np.ascontiguousarray(np.concatenate(wb_all), dtype=np.uint8),
+        cell_row=np.ascontiguousarray(np.concatenate(row_all)),
+        cell_sys=np.ascontiguousarray(np.concatenate(sys_all)),
+        node_start=np.array(starts, dtype=np.int64),
+        ncells=len(sn), nnodes=int(starts[-1]),
+        types=(ids, np.ascontiguousarray(ns, dtype=np.int32), np.ascontiguousarray(h0),
+               np.ascontiguousarray(c0), np.ascontiguousarray(free),
+               np.ascontiguousarray(temp)))
+
+
 class BatchVerlet(object):
     """NVE / NVT-NHC velocity Verlet of many systems in one device engine.
 
@@ -97,44 +138,20 @@ class BatchVerlet(object):
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         # ---------------------------------------------------------- concatenated topology
-        ids, ns, h0, c0, free, temp = _type_table(self.systems)
-        row_of = {t: r for r, t in enumerate(ids)}
-        starts = [0]
-        for s in self.systems:
-            starts.append(starts[-1] + -(-s.nnodes // NODE_ALIGN) * NODE_ALIGN)
-        self.node_start = np.array(starts, dtype=np.int64)
-        Ntot = int(starts[-1])
-        sn_all, sc_all, wb_all, row_all, sys_all = [], [], [], [], []
-        cell_off = 0
-        for k, s in enumerate(self.systems):
-            sn = np.asarray(s.surrounding_nodes, dtype=np.int64) + starts[k]
-            sc = np.asarray(s.surrounding_cells, dtype=np.int64)
-            sc = np.where(sc >= 0, sc + cell_off, -1)
-            pad = np.full((starts[k + 1] - starts[k] - s.nnodes, 8), -1, dtype=np.int64)
-            sn_all.append(sn)
-            sc_all.extend([sc, pad])
-            wb_all.append(system_wrap_bits(s))
-            types = np.rint(np.asarray(s.types, dtype=float)).astype(int)
-            row_all.append(np.array([row_of[int(t)] for t in types], dtype=np.int32))
-            sys_all.append(np.full(s.ncells, k, dtype=np.int32))
-            cell_off += s.ncells
-        sn = np.ascontiguousarray(np.concatenate(sn_all))
-        sc = np.ascontiguousarray(np.concatenate(sc_all))
-        wb = np.ascontiguousarray(np.concatenate(wb_all), dtype=np.uint8)
-        rows = np.ascontiguousarray(np.concatenate(row_all))
-        csys = np.ascontiguousarray(np.concatenate(sys_all))
-        self.ncells_total = len(sn)
+        topo = concat_systems(self.systems, NODE_ALIGN)
+        starts = topo["node_start"]
+        self.node_start = starts
+        Ntot = topo["nnodes"]
+        self.ncells_total = topo["ncells"]
         self.lib = _native.lib()
         handle = ctypes.c_void_p()
         vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-        h0 = np.ascontiguousarray(h0)
-        c0 = np.ascontiguousarray(c0)
-        free = np.ascontiguousarray(free)
-        temp = np.ascontiguousarray(temp)
+        ids, ns, h0, c0, free, temp = topo["types"]
         rc = self.lib.mm_batch_field_create(
-            ctypes.byref(handle), self.device.index, _native.MODELS[model], n, len(sn), Ntot,
-            vp(sn), vp(sc), vp(wb), vp(rows), vp(csys), len(ids), int(h0.shape[1]), vp(ns),
-            vp(h0), vp(c0), vp(free), vp(temp), float(units.boltzmann))
+            ctypes.byref(handle), self.device.index, _native.MODELS[model], n, topo["ncells"],
+            Ntot, vp(topo["surrounding_nodes"]), vp(topo["surrounding_cells"]),
+            vp(topo["wrap_bits"]), vp(topo["cell_row"]), vp(topo["cell_sys"]), len(ids),
+            int(h0.shape[1]), vp(ns), vp(h0), vp(c0), vp(free), vp(temp), float(units.boltzmann))
         _native.check(rc, "mm_batch_field_create")
         self.field = handle
         # ---------------------------------------------------------- host initialisation

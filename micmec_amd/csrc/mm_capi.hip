@@ -72,7 +72,7 @@ void voigt_average(const double* C, double* Cv) {
 int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_t* nstates,
                       const double* h0, const double* c0, const double* free_energy,
                       const double* temp_eff, double boltzmann, std::vector<double>& tab,
-                      std::vector<double>& kt) {
+                      std::vector<double>& kt, bool c_is_voigt) {
   tab.assign((size_t)nrows * max_states * kTabStride, 0.0);
   kt.assign(nrows, 0.0);
   for (int t = 0; t < nrows; ++t) {
@@ -90,7 +90,11 @@ int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_
         return set_error(MM_ERR_ARG, "row %d state %d: singular equilibrium cell matrix", t, s);
       e[9] = model == MM_MODEL_CORNER8 ? 0.125 * det : det;
       e[10] = free_energy[ts];
-      voigt_average(c0 + ts * 81, e + 12);
+      if (c_is_voigt) {
+        for (int q = 0; q < 36; ++q) e[12 + q] = c0[ts * 36 + q];
+      } else {
+        voigt_average(c0 + ts * 81, e + 12);
+      }
     }
   }
   return MM_OK;
@@ -209,7 +213,7 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
   }
   std::vector<double> tab, kt;
   int rc = build_param_table(model, nrows, max_states, nstates, h0, c0, free_energy, temp_eff,
-                             boltzmann, tab, kt);
+                             boltzmann, tab, kt, false);
   if (rc) return rc;
 
   // Work order: cells grouped by (parameter row, system), reference order inside a group, each
@@ -325,6 +329,27 @@ int mm_batch_field_create(mm_field** out, int device, int model, int32_t nsys, i
   return create_field(out, device, model, nsys, ncells, nnodes, surrounding_nodes,
                       surrounding_cells, wrap_bits, cell_row, cell_sys, nrows, max_states,
                       nstates, h0, c0, free_energy, temp_eff, boltzmann);
+}
+
+int mm_field_set_params(mm_field* f, const int32_t* nstates, const double* h0,
+                        const double* cvoigt, const double* free_energy, const double* temp_eff,
+                        double boltzmann, void* stream) {
+  if (!f || !nstates || !h0 || !cvoigt || !free_energy || !temp_eff)
+    return set_error(MM_ERR_ARG, "missing argument");
+  std::vector<double> tab, kt;
+  int rc = build_param_table(f->model, f->nrows, f->max_states, nstates, h0, cvoigt, free_energy,
+                             temp_eff, boltzmann, tab, kt, true);
+  if (rc) return rc;
+  DeviceGuard guard(f->device);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(f->tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(f->kt, kt.data(), sizeof(double) * kt.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(f->nstates, nstates, sizeof(int32_t) * f->nrows, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_error(e, "mm_field_set_params");
+  return MM_OK;
 }
 
 int mm_batch_compute(mm_field* f, const double* pos, const double* rvecs, double* energies,

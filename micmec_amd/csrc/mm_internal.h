@@ -15,7 +15,7 @@ void voigt_average(const double* C, double* Cv);
 int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_t* nstates,
                       const double* h0, const double* c0, const double* free_energy,
                       const double* temp_eff, double boltzmann, std::vector<double>& tab,
-                      std::vector<double>& kt);
+                      std::vector<double>& kt, bool c_is_voigt);
 
 }  // namespace mm
 

@@ -1,0 +1,348 @@
+"""Differentiation form on the GPU: the adjoint kernel against the force path, central finite
+differences (h = 1e-6 relative, agreement to 1e-6 relative), a closed form for the elasticity
+derivative and finite differences of the CPU oracle's energy, and the autograd Function against
+finite differences of a force-and-stress loss.
+
+The reference has no parameter derivatives (micmec/pes/mmff.py:384-398 differentiates with
+respect to positions only); finite differences of the energy, itself pinned against the oracle,
+and the closed form below are what pins them.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from micmec_amd import units
+from micmec_amd.adjoint import (CellParams, DiffField, FitProblem, fit, micmec_energy,
+                                tensor_from_voigt, voigt_pairs)
+from micmec_amd.pes.mmff import system_wrap_bits
+from micmec_amd.workloads import make_system
+from oracle.mmff import compute_vectorised
+
+pytestmark = pytest.mark.gpu
+A = units.angstrom
+RTOL = 1e-6
+
+
+def mixed_systems():
+    return [make_system((4, 3, 5), p_b=0.5, p_hole=0.15, seed=11, noise=0.1 * A, strain=0.03),
+            make_system((3, 3, 3), p_b=0.3, seed=12, noise=0.1 * A, strain=0.02),
+            make_system((2, 4, 3), p_b=0.6, p_hole=0.1, seed=13, noise=0.1 * A, pbc=False)]
+
+
+def competing_states(p):
+    """At 300 K the second state of type B lies about 37 kT above the first (p_2 ~ 1e-16), so
+    the state combination would be inert; at 2e4 K both states carry weight."""
+    p.teff.fill_(2.0e4)
+    return p
+
+
+def params_copy(p):
+    return CellParams(*[x.detach().clone() for x in p.tensors()], p.nstates, p.type_ids)
+
+
+def energies(field, pos, rv, params):
+    field.set_params(params)
+    e, g, v = field.compute(pos, rv)
+    return e
+
+
+def assert_close_rel(a, b, rtol, what=""):
+    a = torch.as_tensor(a).double().cpu()
+    b = torch.as_tensor(b).double().cpu()
+    scale = float(b.abs().max())
+    assert scale > 0.0, "%s: reference is identically zero" % what
+    err = float((a - b).abs().max())
+    assert err <= rtol * scale, "%s: max error %.3e vs scale %.3e" % (what, err, scale)
+
+
+@pytest.fixture(scope="module", params=["corner8", "original"])
+def setup(request):
+    field = DiffField(mixed_systems(), model=request.param)
+    pos, rv = field.positions()
+    params = competing_states(field.initial_params())
+    field.set_params(params)
+    return field, pos, rv, params
+
+
+def test_values_match_force_path(setup):
+    field, pos, rv, params = setup
+    e, g, v = field.compute(pos, rv)
+    out = field.adjoint(pos, rv)
+    assert_close_rel(out["energy"], e, 1e-12, "energy")
+    assert_close_rel(out["gpos"], g, 1e-12, "gpos")
+    assert_close_rel(out["vtens"], v, 1e-12, "vtens")
+    # dual numbers carry the same values
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    dpos = torch.randn(pos.shape, generator=gen, dtype=torch.float64).to(pos.device)
+    drv = torch.randn(rv.shape, generator=gen, dtype=torch.float64).to(pos.device)
+    dual = field.adjoint(pos, rv, dpos, drv)
+    for k in ("energy", "gpos", "vtens", "grvecs", "params"):
+        assert_close_rel(dual[k], out[k], 1e-14, k)
+
+
+def test_domain_gradient_identity(setup):
+    """vtens = sum_n gpos_n (x) pos_n + sum_d (dE/drvecs_d) (x) rvecs_d per system (the virial
+    of all vertices, images included, mmff.py:320-323)."""
+    field, pos, rv, params = setup
+    out = field.adjoint(pos, rv)
+    g, grv = out["gpos"].cpu().numpy(), out["grvecs"].cpu().numpy()
+    p, r = pos.cpu().numpy(), rv.cpu().numpy()
+    for k in range(field.nsys):
+        a, b = field.node_start[k], field.node_start[k + 1]
+        vt = g[a:b].T @ p[a:b] + grv[k].T @ r[k]
+        scale = np.abs(g[a:b]).sum(0)[:, None] * np.abs(p[a:b]).max()
+        assert np.abs(vt - out["vtens"][k].cpu().numpy()).max() <= 1e-11 * scale.max(), k
+
+
+def test_domain_gradient_fd(setup):
+    field, pos, rv, params = setup
+    out = field.adjoint(pos, rv)
+    grv = out["grvecs"].cpu().numpy()
+    for k in range(field.nsys):
+        if field.systems[k].domain.nvec == 0:
+            assert np.all(grv[k] == 0.0)
+            continue
+        fd = np.zeros((3, 3))
+        for d in range(3):
+            h = 1e-6 * float(rv[k, d].norm())
+            for j in range(3):
+                rp, rm = rv.clone(), rv.clone()
+                rp[k, d, j] += h
+                rm[k, d, j] -= h
+                fd[d, j] = float(energies(field, pos, rp, params)[k]
+                                 - energies(field, pos, rm, params)[k]) / (2 * h)
+        assert_close_rel(fd, grv[k], RTOL, "dE/drvecs system %d" % k)
+
+
+def param_fd(field, pos, rv, params, weights):
+    """Central differences of sum_s weights_s E_s with respect to every parameter entry."""
+    T, S = field.nrows, field.max_states
+    w = weights.to(pos.device)
+
+    def L(p):
+        return float((energies(field, pos, rv, p) * w).sum())
+
+    fd_cv = np.zeros((T, S, 6, 6))
+    fd_h0 = np.zeros((T, S, 3, 3))
+    fd_free = np.zeros((T, S))
+    fd_teff = np.zeros(T)
+    for t in range(T):
+        for s in range(int(params.nstates[t])):
+            scale_c = float(params.cv[t, s].abs().max())
+            for I, J in voigt_pairs():
+                h = 1e-6 * scale_c
+                pp, pm = params_copy(params), params_copy(params)
+                for q in {(I, J), (J, I)}:
+                    pp.cv[t, s][q] += h
+                    pm.cv[t, s][q] -= h
+                fd_cv[t, s, I, J] = fd_cv[t, s, J, I] = (L(pp) - L(pm)) / (2 * h)
+            for i in range(3):
+                for j in range(3):
+                    h = 1e-6 * float(params.h0[t, s].abs().max())
+                    pp, pm = params_copy(params), params_copy(params)
+                    pp.h0[t, s, i, j] += h
+                    pm.h0[t, s, i, j] -= h
+                    fd_h0[t, s, i, j] = (L(pp) - L(pm)) / (2 * h)
+            h = 1e-6 * units.boltzmann * float(params.teff[t])
+            pp, pm = params_copy(params), params_copy(params)
+            pp.free[t, s] += h
+            pm.free[t, s] -= h
+            fd_free[t, s] = (L(pp) - L(pm)) / (2 * h)
+        if params.nstates[t] > 1:
+            h = 1e-6 * float(params.teff[t])
+            pp, pm = params_copy(params), params_copy(params)
+            pp.teff[t] += h
+            pm.teff[t] -= h
+            fd_teff[t] = (L(pp) - L(pm)) / (2 * h)
+    return fd_cv, fd_h0, fd_free, fd_teff
+
+
+def test_parameter_gradients_fd(setup):
+    field, pos, rv, params = setup
+    weights = torch.tensor([1.0, -0.5, 2.0], dtype=torch.float64)
+    field.set_params(params)
+    out = field.adjoint(pos, rv, energy_weight=weights.to(pos.device))
+    g_cv, g_h0, g_free, g_teff = field.param_grads(out["params"], params)
+    fd_cv, fd_h0, fd_free, fd_teff = param_fd(field, pos, rv, params, weights)
+    # the symmetric perturbation moves (I, J) and (J, I): compare the matching sums
+    sym = g_cv.cpu().numpy()
+    sym = np.where(np.eye(6, dtype=bool), sym, 2 * sym)
+    assert_close_rel(sym, fd_cv, RTOL, "dE/dCv")
+    assert_close_rel(g_h0, fd_h0, RTOL, "dE/dh0")
+    assert_close_rel(g_free, fd_free, RTOL, "dE/dfree")
+    assert_close_rel(g_teff, fd_teff, RTOL, "dE/dT_eff")
+    for t in range(field.nrows):
+        if params.nstates[t] == 1:
+            assert float(g_teff[t]) == 0.0
+        else:
+            assert abs(fd_teff[t]) > 0.0
+    # states a type does not have get no gradient
+    for t in range(field.nrows):
+        assert torch.all(g_cv[t, int(params.nstates[t]):] == 0)
+
+
+def test_tangents_fd(setup):
+    """Every tangent output equals the central difference of its value along the direction."""
+    field, pos, rv, params = setup
+    field.set_params(params)
+    gen = torch.Generator(device="cpu").manual_seed(9)
+    dpos = torch.randn(pos.shape, generator=gen, dtype=torch.float64).to(pos.device)
+    drv = torch.randn(rv.shape, generator=gen, dtype=torch.float64).to(pos.device)
+    for k, s in enumerate(field.systems):
+        drv[k, s.domain.nvec:] = 0.0
+    h = 1e-6 * float(A)
+    dual = field.adjoint(pos, rv, dpos, drv)
+    plus = field.adjoint((pos + h * dpos).contiguous(), (rv + h * drv).contiguous())
+    minus = field.adjoint((pos - h * dpos).contiguous(), (rv - h * drv).contiguous())
+    for k in ("energy", "gpos", "vtens", "grvecs", "params"):
+        fd = (plus[k] - minus[k]) / (2 * h)
+        assert_close_rel(dual["d" + k], fd, RTOL, "tangent of " + k)
+    # the tangent of the energy is the directional derivative gpos.dpos + grvecs:drvecs
+    ns = field.node_sys
+    lin = torch.zeros(field.nsys, dtype=torch.float64, device=pos.device)
+    lin.index_add_(0, ns, (dual["gpos"] * dpos).sum(1))
+    lin += (dual["grvecs"] * drv).sum((1, 2))
+    assert_close_rel(dual["denergy"], lin, 1e-11, "denergy")
+
+
+def test_closed_form_elasticity_gradient():
+    """Affine strain of a perfect lattice of one single-state type: every corner matrix is
+    h0 F, so E = C det(h0) e.Cv.e / 2 with e the Voigt strain of eps = (F F^T - I) / 2 and
+    dE/dCv_IJ = C det(h0) e_I e_J / 2, dE/d(free) = C."""
+    for model in ("corner8", "original"):
+        s = make_system((3, 4, 5), p_b=0.0, seed=21, noise=0.0, strain=0.02)
+        field = DiffField([s], model=model)
+        pos, rv = field.positions()
+        params = field.initial_params()
+        field.set_params(params)
+        out = field.adjoint(pos, rv)
+        g_cv, g_h0, g_free, g_teff = field.param_grads(out["params"], params)
+        h0 = params.h0[0, 0].cpu().numpy()
+        F = np.linalg.solve(h0, (np.asarray(s.domain.rvecs) / np.array([[3], [4], [5]])))
+        eps = 0.5 * (F @ F.T - np.identity(3))
+        e = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2],
+                      2 * eps[0, 1]])
+        C = s.ncells
+        expect = C * np.linalg.det(h0) * 0.5 * np.outer(e, e)
+        assert_close_rel(g_cv[0, 0], expect, 1e-10, "closed form dE/dCv " + model)
+        cv = params.cv[0, 0].cpu().numpy()
+        assert abs(float(out["energy"][0]) - C * np.linalg.det(h0) * 0.5 * e @ cv @ e) <= \
+            1e-10 * abs(float(out["energy"][0]))
+        assert abs(float(g_free[0, 0]) - C) <= 1e-12 * C
+
+
+def test_parameter_gradient_vs_oracle_fd():
+    """dE/dC0 of the GPU adjoint against central differences of the CPU oracle's energy
+    (restating micmec/pes/mmff.py:326-403) on a 3^3 two-type system."""
+    s = make_system((3, 3, 3), p_b=0.4, seed=31, noise=0.1 * A, strain=0.03)
+    field = DiffField([s])
+    pos, rv = field.positions()
+    params = competing_states(field.initial_params())
+    field.set_params(params)
+    out = field.adjoint(pos, rv)
+    g_cv = field.param_grads(out["params"], params)[0].cpu().numpy()
+    wb = system_wrap_bits(s)
+    cv = params.cv.cpu().numpy()
+
+    def oracle_energy(cv_mod):
+        p = dict(s.params)
+        for r, t in enumerate(params.type_ids):
+            ns = int(params.nstates[r])
+            p["type%i/elasticity" % t] = tensor_from_voigt(cv_mod[r, :ns])
+            p["type%i/effective_temp" % t] = float(params.teff[r])
+        e, _, _ = compute_vectorised(s.pos, s.domain.rvecs, s.types, s.surrounding_nodes,
+                                     s.surrounding_cells, wb, p)
+        return e
+
+    for r in range(field.nrows):
+        for st in range(int(params.nstates[r])):
+            for I, J in [(0, 0), (0, 1), (1, 2), (3, 3), (2, 4)]:
+                h = 1e-6 * np.abs(cv[r, st]).max()
+                cp, cm = cv.copy(), cv.copy()
+                for q in {(I, J), (J, I)}:
+                    cp[r, st][q] += h
+                    cm[r, st][q] -= h
+                fd = (oracle_energy(cp) - oracle_energy(cm)) / (2 * h)
+                adj = g_cv[r, st, I, J] * (1 if I == J else 2)
+                scale = np.abs(g_cv[r, st]).max()
+                assert abs(fd - adj) <= RTOL * scale, (r, st, I, J, fd, adj)
+
+
+def loss_fd_check(problem, params, what, direction, h):
+    """Directional central difference of the loss against the autograd gradient."""
+    leaf = [x.detach().clone().requires_grad_(True) for x in params.tensors()]
+    p = CellParams(*leaf, params.nstates, params.type_ids)
+    pos = problem.pos.detach().clone().requires_grad_(True)
+    rv = problem.rvecs.detach().clone().requires_grad_(True)
+    problem_pos, problem_rv = problem.pos, problem.rvecs
+    problem.pos, problem.rvecs = pos, rv
+    loss = problem.loss(p)
+    loss.backward()
+    grads = dict(cv=leaf[0].grad, h0=leaf[1].grad, free=leaf[2].grad, teff=leaf[3].grad,
+                 pos=pos.grad, rvecs=rv.grad)
+    problem.pos, problem.rvecs = problem_pos, problem_rv
+
+    def L(shift):
+        q = params_copy(params)
+        pp, rr = problem.pos, problem.rvecs
+        if what in ("cv", "h0", "free", "teff"):
+            getattr(q, what).add_(shift * direction)
+        elif what == "pos":
+            problem.pos = (pp + shift * direction).contiguous()
+        else:
+            problem.rvecs = (rr + shift * direction).contiguous()
+        with torch.no_grad():
+            val = float(problem.loss(q))
+        problem.pos, problem.rvecs = pp, rr
+        return val
+
+    fd = (L(h) - L(-h)) / (2 * h)
+    an = float((grads[what] * direction).sum())
+    return an, fd
+
+
+def test_autograd_force_stress_loss():
+    systems = [make_system((3, 3, 4), p_b=0.5, p_hole=0.1, seed=41, noise=0.1 * A, strain=0.03),
+               make_system((3, 3, 3), p_b=0.5, seed=42, noise=0.1 * A, strain=0.03)]
+    field = DiffField(systems)
+    pos, rv = field.positions()
+    target = field.initial_params()
+    target.cv = target.cv * 1.07
+    target = competing_states(target)
+    problem = FitProblem(field, pos, rv, target)
+    params = competing_states(field.initial_params())
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    dev = pos.device
+
+    def rnd(shape):
+        return torch.randn(shape, generator=gen, dtype=torch.float64).to(dev)
+
+    m = rnd(params.cv.shape)
+    cases = [("cv", 0.5 * (m + m.transpose(-1, -2)) * params.cv.abs().max(), 1e-6),
+             ("h0", rnd(params.h0.shape) * float(params.h0.abs().max()), 1e-6),
+             ("free", rnd(params.free.shape) * units.boltzmann * 300, 1e-6),
+             ("teff", rnd(params.teff.shape) * 300.0, 1e-6),
+             ("pos", rnd(pos.shape) * A, 1e-6),
+             ("rvecs", rnd(rv.shape) * A, 1e-6)]
+    for what, direction, h in cases:
+        if what in ("cv", "h0", "free"):
+            for t in range(field.nrows):
+                direction[t, int(params.nstates[t]):] = 0.0
+        an, fd = loss_fd_check(problem, params, what, direction, h)
+        assert abs(an - fd) <= RTOL * max(abs(an), abs(fd)), (what, an, fd)
+
+
+def test_fit_reduces_loss():
+    systems = [make_system((4, 4, 4), p_b=0.3, seed=50 + i, noise=0.1 * A, strain=0.03)
+               for i in range(4)]
+    field = DiffField(systems)
+    pos, rv = field.positions()
+    target = field.initial_params()
+    rng = np.random.default_rng(7)
+    noise = rng.uniform(-0.1, 0.1, tuple(target.cv.shape))
+    noise = 0.5 * (noise + np.swapaxes(noise, -1, -2))
+    target.cv = target.cv * (1.0 + torch.tensor(noise, device=pos.device))
+    problem = FitProblem(field, pos, rv, target)
+    losses, final = fit(problem, field.initial_params(), 60)
+    assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
