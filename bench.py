@@ -50,6 +50,12 @@ def parse():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--sequential-steps", type=int, default=20)
+    ap.add_argument("--adjoint", dest="adjoint", action="store_true", default=True)
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--pmc-probe", type=int, default=0,
+                    help="launch only the cell pass this many times, then copy 1 GiB with a "
+                         "plain copy kernel (the counter calibration), and exit")
     return ap.parse_args()
 
 
@@ -201,6 +207,59 @@ def batched_vs_sequential(count, steps, seq_steps):
             "speedup": value_b / value_s}
 
 
+def pmc_probe(n, model, launches):
+    """Counter probe: the cell pass alone, then a 1 GiB device copy (bytes known exactly)."""
+    import torch
+    system, fpm, verlet = build_workload(n, model)
+    torch.cuda.synchronize()
+    t = kernel_timing(fpm, verlet, launches)
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": {"cell_pass_launches": launches + 3, "cell_pass_ms": t * 1e3,
+                                    "copy_bytes_each_way": a.numel() * 8, "copies": 3}}))
+
+
+def differentiation(steps):
+    """Force-and-stress fit of the elasticity matrices: 64 configurations on 16^3 grids, each
+    Adam step one forward force pass and one dual-number adjoint pass over all of them."""
+    import torch
+    from micmec_amd.adjoint import fit, fit_workload
+    problem, init, target = fit_workload(count=64, n=16)
+    field = problem.field
+    fit(problem, init, 3)                      # warm-up (library plans, allocator)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses, final = fit(problem, init, steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # the adjoint launch alone, HIP events on the current stream
+    pos, rv = problem.pos, problem.rvecs
+    dpos = torch.randn_like(pos)
+    drv = torch.randn_like(rv)
+    field.set_params(final)
+    field.adjoint(pos, rv, dpos, drv)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    ev0.record(stream)
+    for _ in range(reps):
+        field.adjoint(pos, rv, dpos, drv)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_adj = ev0.elapsed_time(ev1) / reps * 1e-3
+    err = float(((final.cv - target.cv).abs().max() / target.cv.abs().max()))
+    return {"workload": "fit64x16", "configurations": field.nsys, "cells": field.ncells,
+            "adam_steps": steps, "ms_per_adam_step": dt / steps * 1e3,
+            "cell_evals_per_s": 2 * field.ncells * steps / dt,
+            "adjoint_call_ms": t_adj * 1e3,
+            "loss_first": losses[0], "loss_last": losses[-1],
+            "cv_max_rel_error_vs_target": err}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -213,6 +272,9 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     n = args.grid
+    if args.pmc_probe > 0:
+        pmc_probe(n, args.model, args.pmc_probe)
+        return
     system, fpm, verlet = build_workload(n, args.model)
     verlet.use_graph = not args.no_graph
     print("workload built: %d cells (%d non-empty), %d nodes" % (n ** 3, system.ncells,
@@ -260,6 +322,9 @@ def main():
         print("batched replicas ...", file=sys.stderr, flush=True)
         out["batched_replicas"] = batched_vs_sequential(args.replicas, args.batched_steps,
                                                         args.sequential_steps)
+    if args.adjoint:
+        print("differentiation fit ...", file=sys.stderr, flush=True)
+        out["differentiation"] = differentiation(args.adjoint_steps)
     if args.cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_grid, args.cpu_seconds, args.model)
     print(json.dumps(out))
