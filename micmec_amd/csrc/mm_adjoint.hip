@@ -210,7 +210,7 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
   for (int s = 0; s < ns; ++s) {
     const double* ts = tab + s * kTabStride;
     const double* P = ts;
-    const double* Cv = ts + 12;
+    const double* Cv = ts + kCvOffset;
     T B[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -265,7 +265,7 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
     const double* ts = tab + s * kTabStride;
     const double* P = ts;
     const double wt = ts[9];
-    const double* Cv = ts + 12;
+    const double* Cv = ts + kCvOffset;
     T ps = make<T>(1.0, 0.0);
     if (ns > 1) {
       const T es = make<T>(es_sh[(s * kAdjCells + cl) * 2], es_sh[(s * kAdjCells + cl) * 2 + 1]);

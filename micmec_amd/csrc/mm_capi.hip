@@ -91,13 +91,30 @@ int build_param_table(int model, int32_t nrows, int32_t max_states, const int32_
       e[9] = model == MM_MODEL_CORNER8 ? 0.125 * det : det;
       e[10] = free_energy[ts];
       if (c_is_voigt) {
-        for (int q = 0; q < 36; ++q) e[12 + q] = c0[ts * 36 + q];
+        for (int q = 0; q < 36; ++q) e[kCvOffset + q] = c0[ts * 36 + q];
       } else {
-        voigt_average(c0 + ts * 81, e + 12);
+        voigt_average(c0 + ts * 81, e + kCvOffset);
       }
+      for (int q = 0; q < 36; ++q) e[kCvwOffset + q] = e[9] * e[kCvOffset + q];
     }
   }
   return MM_OK;
+}
+
+void update_split(mm_field* f, const int32_t* nstates) {
+  const int64_t nb = (int64_t)f->blk_row_host.size();
+  int64_t lead = 0;
+  while (lead < nb && nstates[f->blk_row_host[lead]] > 1) ++lead;
+  bool ok = true;
+  for (int64_t b = lead; b < nb && ok; ++b) ok = nstates[f->blk_row_host[b]] <= 1;
+  f->split = ok ? (int)lead : -1;
+}
+
+void schedule_args(const mm_field* f, ForceArgs& a) {
+  a.split = f->split;
+  a.side = f->side;
+  a.ev_fork = f->ev_fork;
+  a.ev_join = f->ev_join;
 }
 
 }  // namespace mm
@@ -141,6 +158,12 @@ void free_field(mm_field* f) {
   (void)hipFree(f->blk_sys);
   (void)hipFree(f->sys_cblk_ptr);
   (void)hipFree(f->sys_cblk_idx);
+  if (f->side) {
+    (void)hipStreamSynchronize(f->side);
+    (void)hipStreamDestroy(f->side);
+  }
+  if (f->ev_fork) (void)hipEventDestroy(f->ev_fork);
+  if (f->ev_join) (void)hipEventDestroy(f->ev_join);
   (void)hipSetDevice(cur);
   delete f;
 }
@@ -218,6 +241,8 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
 
   // Work order: cells grouped by (parameter row, system), reference order inside a group, each
   // group padded to a whole number of blocks so that every block is type- and system-uniform.
+  // Rows with several states come first: their blocks take longest, and starting them early
+  // lets the single-state blocks fill the end of each launch.
   std::vector<std::vector<int64_t>> groups((size_t)nrows * nsys);
   for (int64_t c = 0; c < ncells; ++c)
     groups[(size_t)cell_row[c] * nsys + (cell_sys ? cell_sys[c] : 0)].push_back(c);
@@ -227,8 +252,13 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
       blk_sys(W / kCellBlock, 0);
   std::vector<int64_t> slot(ncells, -1);
   std::vector<std::vector<int32_t>> sys_blocks(nsys);
+  std::vector<int> row_order;
+  for (int t = 0; t < nrows; ++t)
+    if (nstates[t] > 1) row_order.push_back(t);
+  for (int t = 0; t < nrows; ++t)
+    if (nstates[t] <= 1) row_order.push_back(t);
   int64_t w0 = 0;
-  for (int t = 0; t < nrows; ++t) {
+  for (int t : row_order) {
     for (int sy = 0; sy < nsys; ++sy) {
       const auto& g = groups[(size_t)t * nsys + sy];
       const int64_t nb = (int64_t)((g.size() + kCellBlock - 1) / kCellBlock);
@@ -295,10 +325,15 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
     return rc;
   }
   hipError_t e = hipMemset(f->nanflag, 0, sizeof(int));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     free_field(f);
-    return hip_error(e, "hipMemset");
+    return hip_error(e, "mm_field_create");
   }
+  f->blk_row_host = blk_row;
+  update_split(f, nstates);
   *out = f;
   return MM_OK;
 }
@@ -349,6 +384,7 @@ int mm_field_set_params(mm_field* f, const int32_t* nstates, const double* h0,
   if (e == hipSuccess)
     e = hipMemcpy(f->nstates, nstates, sizeof(int32_t) * f->nrows, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_error(e, "mm_field_set_params");
+  update_split(f, nstates);
   return MM_OK;
 }
 
@@ -381,6 +417,7 @@ int mm_batch_compute(mm_field* f, const double* pos, const double* rvecs, double
   a.accumulate = 0;
   a.model = f->model;
   a.nanflag = f->nanflag;
+  schedule_args(f, a);
   a.force_grad = 1;
   hipError_t e = launch_cells(a, (hipStream_t)stream);
   if (e == hipSuccess && gpos) e = launch_gather(a, (hipStream_t)stream);
@@ -420,6 +457,7 @@ int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double
   a.accumulate = accumulate;
   a.model = f->model;
   a.nanflag = f->nanflag;
+  schedule_args(f, a);
   hipError_t e = launch_force(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "force kernels");
   return MM_OK;
@@ -469,6 +507,7 @@ int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, v
   a.accumulate = 0;
   a.model = f->model;
   a.nanflag = f->nanflag;
+  schedule_args(f, a);
   a.force_grad = 1;
   hipError_t e = launch_cells(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "cell kernel");

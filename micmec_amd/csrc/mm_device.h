@@ -21,7 +21,10 @@ namespace mm {
 //   [10]     free energy of the state
 //   [11]     unused
 //   [12..47] Voigt matrix Cv[6][6] (row-major), averaged over index orderings
-constexpr int kTabStride = 48;
+//   [48..83] w * Cv, the weighted matrix of the force kernels
+constexpr int kTabStride = 84;
+constexpr int kCvOffset = 12;
+constexpr int kCvwOffset = 48;
 
 // Edge endpoints: corner a, axis d -> lower / upper corner of the edge along d through a.
 __device__ constexpr int kLo[8][3] = {
@@ -37,7 +40,7 @@ __device__ constexpr int kOff[8][3] = {
 // M is given row-major (rows = edges). Returns u = e.Cv.e / 2; writes G = dU/dM (unscaled).
 template <bool kGrad>
 __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, const double M[9],
-                                                double G[9]) {
+                                                double G[9], int cv_offset = kCvOffset) {
   const double* P = tab;
   double B[9];
 #pragma unroll
@@ -55,7 +58,7 @@ __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, 
   e[3] = B[3] * B[6] + B[4] * B[7] + B[5] * B[8];
   e[4] = B[0] * B[6] + B[1] * B[7] + B[2] * B[8];
   e[5] = B[0] * B[3] + B[1] * B[4] + B[2] * B[5];
-  const double* Cv = tab + 12;
+  const double* Cv = tab + cv_offset;
   double s[6];
 #pragma unroll
   for (int I = 0; I < 6; ++I) {
@@ -87,13 +90,13 @@ __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, 
 }
 
 // Energy of one metastable state of a cell (without the free-energy offset) and, if kGrad,
-// the gradient with respect to the eight vertices, accumulated (scaled by w) into g[8][3].
+// the gradient with respect to the eight vertices, accumulated (scaled by gscale) into g[8][3].
+// The weight w enters through the weighted Voigt matrix w Cv, so the corner terms come out
+// already multiplied by w. kUnit: gscale is 1 (single-state cells) and is not applied.
 // kModel: 0 = corner8 (nanocell.py), 1 = original (nanocell_original.py).
-template <int kModel, bool kGrad>
+template <int kModel, bool kGrad, bool kUnit = false>
 __device__ __forceinline__ double state_energy(const double* __restrict__ tab, const double r[8][3],
                                                double g[8][3], double gscale = 1.0) {
-  const double w = tab[9];
-  const double wg = w * gscale;
   if (kModel == 0) {
     double usum = 0.0;
 #pragma unroll
@@ -104,20 +107,20 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
 #pragma unroll
         for (int j = 0; j < 3; ++j) M[3 * d + j] = r[kHi[a][d]][j] - r[kLo[a][d]][j];
       }
-      usum += corner_energy<kGrad>(tab, M, G);
+      usum += corner_energy<kGrad>(tab, M, G, kCvwOffset);
       if (kGrad) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
 #pragma unroll
           for (int j = 0; j < 3; ++j) {
-            const double v = wg * G[3 * d + j];
+            const double v = kUnit ? G[3 * d + j] : gscale * G[3 * d + j];
             g[kHi[a][d]][j] += v;
             g[kLo[a][d]][j] -= v;
           }
         }
       }
     }
-    return w * usum;
+    return usum;
   } else {
     // H = (1/8) sum_a M_a: row d is a quarter of the sum of the four edges along d.
     double H[9], G[9];
@@ -131,19 +134,22 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
         H[3 * d + j] = 0.125 * acc;
       }
     }
-    const double u = corner_energy<kGrad>(tab, H, G);
+    const double u = corner_energy<kGrad>(tab, H, G, kCvwOffset);
     if (kGrad) {
+      const double q = kUnit ? 0.25 : 0.25 * gscale;
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const double sgn = kOff[v][d] ? 0.25 : -0.25;
 #pragma unroll
-          for (int j = 0; j < 3; ++j) g[v][j] += sgn * wg * G[3 * d + j];
+          for (int j = 0; j < 3; ++j) {
+            const double t = q * G[3 * d + j];
+            g[v][j] += kOff[v][d] ? t : -t;
+          }
         }
       }
     }
-    return w * u;
+    return u;
   }
 }
 

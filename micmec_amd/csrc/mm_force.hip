@@ -14,9 +14,7 @@
 // parameter row are uniform per block and the parameters are read with scalar loads.
 //
 // One lane per cell: the lane gathers the cell's eight nodes, evaluates the corner matrices of
-// every state and accumulates the eight vertex gradients in registers. Blocks of single-state
-// types and of multistate types run in separate launches so that the single-state kernel does
-// not carry the registers of the state combination.
+// every state and accumulates the eight vertex gradients in registers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -71,9 +69,12 @@ __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
   }
 }
 
-// One lane per cell. Energies of all states of the block's cell type, combined with a running
-// log-sum-exp (mmff.py:386-398); the vertex gradient of the combined energy goes to gcell and
-// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials.
+// One lane per cell. Energies of all states of the block's cell type, combined by the
+// log-sum-exp of mmff.py:386-398; the vertex gradient of the combined energy goes to gcell and
+// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials. A block holds
+// one cell type; single-state and multistate blocks run in separate instances (kMulti), each
+// launched over its own range of blocks (blk0 + blockIdx.x) when the work order allows, so that
+// the single-state instance does not carry the registers of the state combination.
 template <int kModel, bool kGrad, bool kMulti>
 __global__ void __launch_bounds__(kCellBlock, 2)
 cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
@@ -82,21 +83,23 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
             int32_t max_states, int64_t W, double* __restrict__ gcell,
-            double* __restrict__ partials) {
+            double* __restrict__ partials, int blk0, int nblocks) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[kMulti ? kMaxStates * kCellBlock : 1];
-  const int trow = blk_row[blockIdx.x];
-  // single-state and multistate blocks run in separate launches (see launch_cells)
+  const int bid = blk0 + (int)blockIdx.x;
+  const int trow = blk_row[bid];
+  // a block of the other kind (full-range launches only) leaves its partials to the other
+  // instance
   if (kMulti != (nstates_g[trow] > 1)) return;
   const int ns = nstates_g[trow];
   const double kt = kt_g[trow];
   const double* tab = tab_g + (int64_t)trow * max_states * kTabStride;
-  const double* rvb = rvecs + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * rvecs_stride : 0);
+  const double* rvb = rvecs + (blk_sys ? (int64_t)blk_sys[bid] * rvecs_stride : 0);
   double rv[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) rv[q] = rvb[q];
 
-  const int64_t w = (int64_t)blockIdx.x * kCellBlock + threadIdx.x;
+  const int64_t w = (int64_t)bid * kCellBlock + threadIdx.x;
   int n[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) n[k] = cnode[w * 8 + k];
@@ -114,7 +117,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       for (int j = 0; j < 3; ++j) gacc[v][j] = 0.0;
     double ecell;
     if (!kMulti) {
-      ecell = tab[10] + state_energy<kModel, kGrad>(tab, r, gacc);
+      ecell = tab[10] + state_energy<kModel, kGrad, true>(tab, r, gacc);
     } else {
       // pass 1: energies of all states, E = E_min - kT ln sum exp(-(E_s - E_min)/kT)
       // (mmff.py:386-395); the state energies are kept in LDS
@@ -156,7 +159,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       }
     }
   }
-  block_reduce_store<10, kCellBlock>(acc, red, partials + blockIdx.x, gridDim.x);
+  block_reduce_store<10, kCellBlock>(acc, red, partials + bid, nblocks);
 }
 
 // gpos[n] (+)= sum_{k=0..7} gcell[k][:][w(n,k)] in fixed k order (mmff.py:303-318).
@@ -208,22 +211,43 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   const int nb = (int)(a.W / kCellBlock);
   const bool grad = a.gpos != nullptr || a.vtens != nullptr || a.force_grad;
   if (nb == 0) return hipSuccess;
-  // Both launches cover every block; each block returns at once unless its type matches the
-  // launch, so the block partials are written exactly once per block.
-#define MM_LAUNCH_CELL(MODEL, GRAD, MULTI)                                                   \
-  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(nb), dim3(kCellBlock), 0,       \
-                     stream, a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap,    \
-                     a.blk_row, a.tab, a.kt,                                                 \
-                     a.nstates, a.max_states, a.W, a.gcell, a.partials)
-  if (a.model == 0) {
-    if (grad) { MM_LAUNCH_CELL(0, true, false); MM_LAUNCH_CELL(0, true, true); }
-    else { MM_LAUNCH_CELL(0, false, false); MM_LAUNCH_CELL(0, false, true); }
-  } else {
-    if (grad) { MM_LAUNCH_CELL(1, true, false); MM_LAUNCH_CELL(1, true, true); }
-    else { MM_LAUNCH_CELL(1, false, false); MM_LAUNCH_CELL(1, false, true); }
+  // Multistate blocks [0, split), single-state blocks [split, nb) when the work order is so
+  // partitioned (split >= 0); otherwise both instances cover every block.
+  const bool part = a.split >= 0;
+  const int m0 = 0, mn = part ? a.split : nb;
+  const int s0 = part ? a.split : 0, sn = part ? nb - a.split : nb;
+  // the multistate instance runs on the side stream, concurrently with the single-state one
+  const bool fork = a.side != nullptr && a.ev_fork != nullptr && a.ev_join != nullptr &&
+                    mn > 0 && sn > 0;
+  hipStream_t ms = fork ? a.side : stream;
+  hipError_t e = hipSuccess;
+  if (fork) {
+    e = hipEventRecord(a.ev_fork, stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(a.side, a.ev_fork, 0);
+    if (e != hipSuccess) return e;
   }
+#define MM_LAUNCH_CELL(MODEL, GRAD, MULTI, S, B0, NBL)                                       \
+  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(NBL), dim3(kCellBlock), 0, S,   \
+                     a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
+                     a.tab, a.kt, a.nstates, a.max_states, a.W, a.gcell, a.partials, B0, nb)
+#define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
+  do {                                                                                       \
+    if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \
+    if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, stream, s0, sn);                         \
+  } while (0)
+  if (a.model == 0) {
+    if (grad) MM_LAUNCH_PAIR(0, true); else MM_LAUNCH_PAIR(0, false);
+  } else {
+    if (grad) MM_LAUNCH_PAIR(1, true); else MM_LAUNCH_PAIR(1, false);
+  }
+#undef MM_LAUNCH_PAIR
 #undef MM_LAUNCH_CELL
-  return hipGetLastError();
+  e = hipGetLastError();
+  if (e == hipSuccess && fork) {
+    e = hipEventRecord(a.ev_join, a.side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, a.ev_join, 0);
+  }
+  return e;
 }
 
 hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {

@@ -40,5 +40,17 @@ struct mm_field {
   int32_t* sys_cblk_ptr = nullptr;    // [nsys + 1] CSR of the cell blocks of every system
   int32_t* sys_cblk_idx = nullptr;    // [W/256]
   std::vector<int64_t> slot_of_cell;  // reference cell index -> work slot
+  std::vector<int32_t> blk_row_host;  // [W/256] copy of blk_row
+  int split = -1;                     // see ForceArgs::split
+  hipStream_t side = nullptr;         // multistate instance of the cell pass runs here
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
+
+namespace mm {
+// Recompute mm_field::split from the state counts of the rows (host array).
+void update_split(mm_field* f, const int32_t* nstates);
+// Fill the cell-pass scheduling fields of a ForceArgs from the field.
+struct ForceArgs;
+void schedule_args(const mm_field* f, ForceArgs& a);
+}  // namespace mm
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -23,8 +24,12 @@
 
 namespace mm {
 
-constexpr int kNodeBlock = 256;
+constexpr int kNodeBlock = 256;   // nodes per block (system boundaries are block aligned)
+constexpr int kNodeThreads = 64;  // one wave per block, kNodeBlock / kNodeThreads nodes a lane
 constexpr int kNodePartial = 16;  // [0] ekin [1..9] sum m v(x)v [10] sum g^2 [11] sum d^2 [12] nan
+constexpr int kScalarThreads = 256;
+constexpr int kScalarSlices = 32;  // most blocks a scalar phase spreads its reduction over
+constexpr int kSliceQ = 23;        // force (10) and node (13) sums per slice
 
 enum NodeMode { kKickDrift = 0, kScalePos = 1, kScaleVel = 2, kVV = 3, kApplyV = 4, kRmsd = 5 };
 
@@ -45,9 +50,19 @@ enum Phase {
 
 // ----------------------------------------------------------------------------- node kernels
 
-// Block totals of a node pass, written SoA: out[q * gridDim.x + blockIdx.x].
+// Block totals of a node pass (one wave per block), written SoA: out[q * gridDim.x + blockIdx.x].
 template <int NV>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* part) {
+  wave_reduce<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[(int64_t)q * gridDim.x + blockIdx.x] = v[q];
+  }
+}
+
+// The same for blocks of kNodeBlock threads (waves summed in order through LDS).
+template <int NV>
+__device__ __forceinline__ void block_sum_store_wide(double (&v)[NV], double* part) {
   __shared__ double red[NV * (kNodeBlock / 64)];
   wave_reduce<NV>(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -105,18 +120,20 @@ __device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, d
 }
 
 template <int kMode>
-__global__ void __launch_bounds__(kNodeBlock)
+__global__ void __launch_bounds__(kNodeThreads)
 node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ g,
             const double* __restrict__ m, const double* __restrict__ xold,
             const double* __restrict__ S0, const int32_t* __restrict__ blk_sys, int64_t N,
             double* __restrict__ partials) {
 #pragma clang fp contract(off)
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
-  const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
   double acc[13];
 #pragma unroll
   for (int q = 0; q < 13; ++q) acc[q] = 0.0;
-  if (n < N) {
+#pragma unroll
+  for (int it = 0; it < kNodeBlock / kNodeThreads; ++it) {
+    const int64_t n = (int64_t)blockIdx.x * kNodeBlock + it * kNodeThreads + threadIdx.x;
+    if (n >= N) break;
     if (kMode == kKickDrift) {
       const double dt = S[MM_S_TIMESTEP];
       const double mass = m[n];
@@ -165,6 +182,7 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 
 // gpos[n] = sum_k gcell[k][:, w(n,k)] (mmff.py:303-318), fused with the second half kick
 // v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities.
+// One node per thread (the gathers need every wave in flight that they can get).
 template <bool kKick>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
@@ -172,11 +190,12 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
                    const double* __restrict__ m, const double* __restrict__ S0,
                    const int32_t* __restrict__ blk_sys, double* __restrict__ partials) {
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
-  const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
   double acc[13];
 #pragma unroll
   for (int q = 0; q < 13; ++q) acc[q] = 0.0;
-  if (n < N) {
+  {
+    const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+    if (n < N) {
     double g[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -206,8 +225,22 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       }
       node_vv(vv, mass, acc);
     }
+    }
   }
-  block_sum_store<13>(acc, partials);
+  if (kKick) {
+    block_sum_store_wide<13>(acc, partials);
+  } else {
+    // only the NaN flag: any lane's flag, per block
+    const unsigned long long bad = __ballot(acc[12] != 0.0);
+    __shared__ int any_bad;
+    if (threadIdx.x == 0) any_bad = 0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && bad) any_bad = 1;
+    __syncthreads();
+    if (threadIdx.x < 13)
+      partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] =
+          (threadIdx.x == 12 && any_bad) ? 1.0 : 0.0;
+  }
 }
 
 // ----------------------------------------------------------------------------- scalar math
@@ -489,18 +522,19 @@ __device__ void take_nodes(double* S, const double (&nd)[13]) {
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
 }
 
-// Sum SoA partials part[q * nb + b] over the blocks b of one system (a list or a range) in a
-// fixed order; every thread receives the totals.
+// Sum SoA partials part[q * nb + b] over the blocks b = idx[first + i] (or first + i),
+// i < count, in a fixed order; every thread of the block receives the totals.
 template <int NV>
 __device__ void reduce_blocks(const double* __restrict__ part, int nb,
                               const int32_t* __restrict__ idx, int first, int count,
                               double (&out)[NV]) {
-  __shared__ double red[NV * 16];
+  constexpr int kWaves = kScalarThreads / 64;
+  __shared__ double red[NV * kWaves];
   __shared__ double tot[NV];
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int i = threadIdx.x; i < count; i += 1024) {
+  for (int i = threadIdx.x; i < count; i += kScalarThreads) {
     const int b = idx ? idx[first + i] : first + i;
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] += part[(int64_t)q * nb + b];
@@ -514,7 +548,7 @@ __device__ void reduce_blocks(const double* __restrict__ part, int nb,
   __syncthreads();
   if (threadIdx.x < NV) {
     double t = red[threadIdx.x];
-    for (int w = 1; w < 16; ++w) t += red[w * NV + threadIdx.x];
+    for (int w = 1; w < kWaves; ++w) t += red[w * NV + threadIdx.x];
     tot[threadIdx.x] = t;
   }
   __syncthreads();
@@ -523,29 +557,19 @@ __device__ void reduce_blocks(const double* __restrict__ part, int nb,
   __syncthreads();
 }
 
-// One block per system: reduce that system's partials, then run the phase on its scalars.
-__global__ void __launch_bounds__(1024)
-scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpart, int nbf,
-              const int32_t* __restrict__ fsys_ptr, const int32_t* __restrict__ fsys_idx,
-              const double* __restrict__ npart, int nbn, const int32_t* __restrict__ nsys_ptr,
-              int with_vtens) {
-  const int sys = blockIdx.x;
-  double* S = S0 + (int64_t)sys * MM_S_COUNT;
-  double f[10], nd[13];
-  const bool use_f = phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
-                     phase == kPhBaroMid || phase == kPhMainEndNpt;
-  const bool use_n = phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
-                     phase == kPhBaroEndPre || phase == kPhMainEndNpt ||
-                     phase == kPhBaroEndPost || phase == kPhRmsd;
-  if (use_f) {
-    const int b0 = fsys_ptr[sys];
-    reduce_blocks<10>(fpart, nbf, fsys_idx, b0, fsys_ptr[sys + 1] - b0, f);
-  }
-  if (use_n) {
-    const int b0 = nsys_ptr[sys];
-    reduce_blocks<13>(npart, nbn, nullptr, b0, nsys_ptr[sys + 1] - b0, nd);
-  }
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ bool phase_uses_force(int phase) {
+  return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
+         phase == kPhBaroMid || phase == kPhMainEndNpt;
+}
+__device__ __forceinline__ bool phase_uses_nodes(int phase) {
+  return phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
+         phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
+         phase == kPhRmsd;
+}
+
+// The scalar update of one phase on the scalar block S (thread 0 of the system's last block).
+__device__ void run_phase(int phase, double* S, const double (&f)[10], const double (&nd)[13],
+                          int with_vtens) {
   const bool thermo = S[MM_S_THERMO] != 0.0;
   switch (phase) {
     case kPhInitForce:
@@ -647,6 +671,68 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
   }
 }
 
+// Grid (slices, systems). Block (g, sys) reduces slice g of that system's force and node
+// partials; the last block of the system to finish (counted with an atomic counter, which it
+// resets) sums the slices in slice order and runs the phase on a copy of the system's scalar
+// block in LDS, written back at the end.
+__global__ void __launch_bounds__(kScalarThreads)
+scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpart, int nbf,
+              const int32_t* __restrict__ fsys_ptr, const int32_t* __restrict__ fsys_idx,
+              const double* __restrict__ npart, int nbn, const int32_t* __restrict__ nsys_ptr,
+              int with_vtens, double* __restrict__ slices, unsigned int* __restrict__ counters) {
+  __shared__ double Sl[MM_S_COUNT];
+  __shared__ double tot[kSliceQ];
+  __shared__ int is_last;
+  const int sys = blockIdx.y;
+  const int G = gridDim.x, gi = blockIdx.x;
+  double f[10], nd[13];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) f[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 13; ++q) nd[q] = 0.0;
+  if (phase_uses_force(phase)) {
+    const int b0 = fsys_ptr[sys], cnt = fsys_ptr[sys + 1] - b0;
+    const int lo = (int)((int64_t)cnt * gi / G), hi = (int)((int64_t)cnt * (gi + 1) / G);
+    reduce_blocks<10>(fpart, nbf, fsys_idx, b0 + lo, hi - lo, f);
+  }
+  if (phase_uses_nodes(phase)) {
+    const int b0 = nsys_ptr[sys], cnt = nsys_ptr[sys + 1] - b0;
+    const int lo = (int)((int64_t)cnt * gi / G), hi = (int)((int64_t)cnt * (gi + 1) / G);
+    reduce_blocks<13>(npart, nbn, nullptr, b0 + lo, hi - lo, nd);
+  }
+  if (G > 1) {
+    if (threadIdx.x == 0) {
+      double* sl = slices + ((int64_t)sys * G + gi) * kSliceQ;
+      for (int q = 0; q < 10; ++q) sl[q] = f[q];
+      for (int q = 0; q < 13; ++q) sl[10 + q] = nd[q];
+      __threadfence();
+      const unsigned int prev = atomicAdd(&counters[sys], 1u);
+      is_last = prev == (unsigned int)(G - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    if (threadIdx.x < kSliceQ) {
+      const double* sl = slices + (int64_t)sys * G * kSliceQ;
+      double t = 0.0;
+      for (int k = 0; k < G; ++k) t += sl[k * kSliceQ + threadIdx.x];
+      tot[threadIdx.x] = t;
+    }
+    if (threadIdx.x == 0) counters[sys] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 10; ++q) f[q] = tot[q];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) nd[q] = tot[10 + q];
+  }
+  double* Sg = S0 + (int64_t)sys * MM_S_COUNT;
+  for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sl[q] = Sg[q];
+  __syncthreads();
+  if (threadIdx.x == 0) run_phase(phase, Sl, f, nd, with_vtens);
+  __syncthreads();
+  for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sg[q] = Sl[q];
+}
+
 }  // namespace mm
 
 using namespace mm;
@@ -666,6 +752,9 @@ struct mm_md {
   int32_t nsys = 1;
   int32_t* node_blk_sys = nullptr;   // [nbn] system of every node block
   int32_t* sys_nblk = nullptr;       // [nsys + 1] node-block range of every system
+  int slices = 1;                    // blocks per system of a reducing scalar phase
+  double* slice_buf = nullptr;       // [nsys][slices][kSliceQ]
+  unsigned int* counters = nullptr;  // [nsys] last-block counters (kept at zero between calls)
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -711,11 +800,12 @@ ForceArgs force_args(mm_md* md) {
   a.model = f->model;
   a.nanflag = f->nanflag;
   a.force_grad = 1;
+  schedule_args(f, a);
   return a;
 }
 
 hipError_t node(mm_md* md, int mode, hipStream_t s) {
-  const dim3 grid(md->nbn), block(kNodeBlock);
+  const dim3 grid(md->nbn), block(kNodeThreads);
 #define MM_NODE(M)                                                                          \
   hipLaunchKernelGGL((node_kernel<M>), grid, block, 0, s, md->pos, md->vel, md->gpos,        \
                      md->masses, md->posold, md->S, md->node_blk_sys, md->N, md->npart)
@@ -731,11 +821,19 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
   return hipGetLastError();
 }
 
+bool reduces(int phase) {
+  return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
+         phase == kPhBaroMid || phase == kPhMainEndNpt || phase == kPhInitProps ||
+         phase == kPhBaroEndPre || phase == kPhBaroEndPost || phase == kPhRmsd;
+}
+
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
   const int nbf = (int)(md->field->W / kCellBlock);
-  hipLaunchKernelGGL(scalar_kernel, dim3(md->nsys), dim3(1024), 0, s, phase, md->S,
+  const int G = reduces(phase) ? md->slices : 1;
+  hipLaunchKernelGGL(scalar_kernel, dim3(G, md->nsys), dim3(kScalarThreads), 0, s, phase, md->S,
                      md->field->partials, nbf, md->field->sys_cblk_ptr,
-                     md->field->sys_cblk_idx, md->npart, md->nbn, md->sys_nblk, with_vtens);
+                     md->field->sys_cblk_idx, md->npart, md->nbn, md->sys_nblk, with_vtens,
+                     md->slice_buf, md->counters);
   return hipGetLastError();
 }
 
@@ -852,7 +950,21 @@ int mm_md_create_batch(mm_md** out, mm_field* field, int32_t nsys, const int64_t
   md->N = field->N;
   md->nbn = nbn_total > 0 ? nbn_total : 1;
   md->nsys = nsys;
-  hipError_t e = hipMalloc((void**)&md->npart, sizeof(double) * kNodePartial * md->nbn);
+  // slices of a reducing scalar phase: about 256 partial blocks each, at most kScalarSlices
+  std::vector<int32_t> cptr(nsys + 1, 0);
+  hipError_t e = hipMemcpy(cptr.data(), field->sys_cblk_ptr, sizeof(int32_t) * (nsys + 1),
+                           hipMemcpyDeviceToHost);
+  int most = 0;
+  for (int s = 0; s < nsys; ++s) {
+    most = std::max(most, cptr[s + 1] - cptr[s]);
+    most = std::max(most, sys_nblk[s + 1] - sys_nblk[s]);
+  }
+  md->slices = std::min(kScalarSlices, std::max(1, (most + 255) / 256));
+  if (e == hipSuccess) e = hipMalloc((void**)&md->npart, sizeof(double) * kNodePartial * md->nbn);
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&md->slice_buf, sizeof(double) * kSliceQ * md->slices * nsys);
+  if (e == hipSuccess) e = hipMalloc((void**)&md->counters, sizeof(unsigned int) * nsys);
+  if (e == hipSuccess) e = hipMemset(md->counters, 0, sizeof(unsigned int) * nsys);
   if (e == hipSuccess) e = hipMalloc((void**)&md->node_blk_sys, sizeof(int32_t) * md->nbn);
   if (e == hipSuccess) e = hipMalloc((void**)&md->sys_nblk, sizeof(int32_t) * (nsys + 1));
   if (e == hipSuccess)
@@ -974,6 +1086,8 @@ void mm_md_destroy(mm_md* md) {
   if (md->ev_in) (void)hipEventDestroy(md->ev_in);
   if (md->ev_out) (void)hipEventDestroy(md->ev_out);
   if (md->npart) (void)hipFree(md->npart);
+  if (md->slice_buf) (void)hipFree(md->slice_buf);
+  if (md->counters) (void)hipFree(md->counters);
   if (md->node_blk_sys) (void)hipFree(md->node_blk_sys);
   if (md->sys_nblk) (void)hipFree(md->sys_nblk);
   delete md;
