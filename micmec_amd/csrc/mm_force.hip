@@ -69,6 +69,16 @@ __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
   }
 }
 
+// Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
+// every XCD has its own L2. Neighbouring segments share nodes, so each XCD is given one
+// contiguous range of the n blocks: block b -> (b mod 8) * (n / 8) + min(b mod 8, n mod 8) + b / 8.
+// The map is a bijection of [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n / kXcds, r = n % kXcds, x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // One lane per cell. Energies of all states of the block's cell type, combined by the
 // log-sum-exp of mmff.py:386-398; the vertex gradient of the combined energy goes to gcell and
 // the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials. A block holds
@@ -86,7 +96,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             double* __restrict__ partials, int blk0, int nblocks) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[kMulti ? kMaxStates * kCellBlock : 1];
-  const int bid = blk0 + (int)blockIdx.x;
+  const int bid = blk0 + xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int trow = blk_row[bid];
   // a block of the other kind (full-range launches only) leaves its partials to the other
   // instance
