@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--adjoint", dest="adjoint", action="store_true", default=True)
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false")
     ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--cg", dest="cg", action="store_true", default=True)
+    ap.add_argument("--no-cg", dest="cg", action="store_false")
+    ap.add_argument("--cg-steps", type=int, default=200)
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="launch only the cell pass this many times, then copy 1 GiB with a "
                          "plain copy kernel (the counter calibration), and exit")
@@ -223,6 +226,32 @@ def pmc_probe(n, model, launches):
                                     "copy_bytes_each_way": a.numel() * 8, "copies": 3}}))
 
 
+def cg_relaxation(n, steps, model):
+    """CG geometry optimisation of the 128^3 system from a 2% random strain (positions only,
+    domain fixed): `steps` iterations, thresholds tight enough that all of them run."""
+    import torch
+    from micmec_amd.pes import ForcePartMechanical, MicMecForceField
+    from micmec_amd.sampling import CartesianDOF, CGOptimizer
+    from micmec_amd.workloads import make_system
+    system = make_system((n, n, n), p_b=0.2, p_hole=0.05, seed=2, strain=0.02)
+    mmf = MicMecForceField(system, [ForcePartMechanical(system, model=model)])
+    dof = CartesianDOF(mmf, gpos_rms=1e-30, dpos_rms=1e-30)
+    opt = CGOptimizer(dof)
+    e0 = opt.epot
+    torch.cuda.synchronize()
+    calls0 = dof._device.calls
+    t0 = time.perf_counter()
+    opt.run(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    evals = dof._device.calls - calls0
+    return {"workload": "cg128", "grid": [n, n, n], "iterations": opt.counter,
+            "force_evals": evals, "ms_per_iteration": dt / max(opt.counter, 1) * 1e3,
+            "ms_per_force_eval": dt / max(evals, 1) * 1e3,
+            "cell_evals_per_s": n ** 3 * evals / dt, "energy_first": e0,
+            "energy_last": opt.epot, "gpos_rms_last": float(dof.gpos_rms)}
+
+
 def differentiation(steps):
     """Force-and-stress fit of the elasticity matrices: 64 configurations on 16^3 grids, each
     Adam step one forward force pass and one dual-number adjoint pass over all of them."""
@@ -322,6 +351,9 @@ def main():
         print("batched replicas ...", file=sys.stderr, flush=True)
         out["batched_replicas"] = batched_vs_sequential(args.replicas, args.batched_steps,
                                                         args.sequential_steps)
+    if args.cg:
+        print("CG relaxation ...", file=sys.stderr, flush=True)
+        out["cg_relaxation"] = cg_relaxation(n, args.cg_steps, args.model)
     if args.adjoint:
         print("differentiation fit ...", file=sys.stderr, flush=True)
         out["differentiation"] = differentiation(args.adjoint_steps)

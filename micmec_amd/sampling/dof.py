@@ -5,6 +5,9 @@
 When the force field is one ``ForcePartMechanical`` (plus optionally a ``ForcePartPressure``),
 ``fun`` also accepts device tensors and evaluates the energy and gradient with the HIP kernels
 without host round trips; numpy input follows the reference path through ``mmf.compute``.
+On the device path the convergence measures are reduced on the GPU, and the host positions and
+gradient (and the system's positions) are refreshed at convergence, before host hooks and at
+the end of a run (``sync_from_device``), not after every iteration.
 """
 
 import numpy as np
@@ -30,8 +33,10 @@ class _DeviceEnergy(object):
         self.energy = torch.zeros(1, **kw)
         self.gpos = torch.zeros((mmf.system.nnodes, 3), **kw)
         self.rvecs = self.fpm.rvecs_device()
+        self.calls = 0
 
     def __call__(self, pos, do_gradient=True):
+        self.calls += 1
         self.fpm.compute_device(pos, self.rvecs, self.energy, self.gpos if do_gradient else None)
         e = float(self.energy.item())
         if np.isnan(e):
@@ -87,6 +92,7 @@ class CartesianDOF(DOF):
         self.select = select
         DOF.__init__(self, mmf)
         self._last_pos = None
+        self._last_pos_dev = None
         try:
             self._device = _DeviceEnergy(mmf)
         except NotImplementedError:
@@ -147,8 +153,40 @@ class CartesianDOF(DOF):
         if getattr(self, "_gpos_dev", None) is not None:
             self._gpos[:] = self._gpos_dev.cpu().numpy()
 
+    def _device_measures(self):
+        """gpos_max, gpos_rms, dpos_max, dpos_rms of the device state (one small copy)."""
+        pos, g = self._pos_dev, self._gpos_dev
+        if self.select is not None:
+            sel = torch.as_tensor(self.select, device=pos.device)
+            pos, g = pos[sel], g[sel]
+        gsq = (g * g).sum(dim=1)
+        first = self._last_pos_dev is None
+        if first:
+            dsq = torch.zeros_like(gsq)
+        else:
+            d = pos - self._last_pos_dev
+            dsq = (d * d).sum(dim=1)
+        m = torch.stack([gsq.max(), gsq.mean(), dsq.max(), dsq.mean()]).sqrt().cpu().numpy()
+        self._last_pos_dev = pos.detach().clone()
+        return first, m
+
     def check_convergence(self):
-        """Convergence criteria of dof.py:155-193."""
+        """Convergence criteria of dof.py:155-193. On the device path the measures are reduced
+        on the GPU; the host arrays are refreshed by ``sync_from_device`` when needed."""
+        if getattr(self, "_pos_dev", None) is not None and getattr(self, "_gpos_dev", None) \
+                is not None:
+            first, (self.gpos_max, self.gpos_rms, self.dpos_max, self.dpos_rms) = \
+                self._device_measures()
+            if first:
+                self.converged = False
+                self.conv_val = 2
+                self.conv_worst = "first_step"
+                self.conv_count = -1
+                return
+            self._finish_convergence()
+            if self.converged:
+                self.sync_from_device()
+            return
         self.sync_from_device()
         if self._last_pos is None:
             self._last_pos = self._pos.copy()
@@ -167,6 +205,10 @@ class CartesianDOF(DOF):
         dpossq = (d ** 2).sum(axis=1)
         self.dpos_max = np.sqrt(dpossq.max())
         self.dpos_rms = np.sqrt(dpossq.mean())
+        self._last_pos[:] = self._pos
+        self._finish_convergence()
+
+    def _finish_convergence(self):
         conv_vals = []
         if self.th_gpos_rms is not None:
             conv_vals.append((self.gpos_rms / self.th_gpos_rms, "gpos_rms"))
@@ -179,4 +221,3 @@ class CartesianDOF(DOF):
         self.conv_val, self.conv_worst = max(conv_vals)
         self.conv_count = sum(int(v >= 1) for v, n in conv_vals)
         self.converged = self.conv_count == 0
-        self._last_pos[:] = self._pos[:]

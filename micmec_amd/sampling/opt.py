@@ -75,6 +75,15 @@ class BaseOptimizer(Iterative):
         Iterative.propagate(self)
         return self.dof.converged
 
+    def _sync_for_hooks(self):
+        # device-resident degrees of freedom: host positions and gradient for the hooks
+        sync = getattr(self.dof, "sync_from_device", None)
+        if sync is not None:
+            sync()
+
+    def finalize(self):
+        self._sync_for_hooks()
+
 
 def _dot(a, b):
     if isinstance(a, torch.Tensor):
