@@ -41,8 +41,6 @@
 #include "mm_internal.h"
 #include "mm_kernels.h"
 
-#pragma clang fp contract(off)
-
 namespace mm {
 
 struct Dual {
@@ -114,21 +112,30 @@ __device__ __forceinline__ void stage(double* row, int q, T x, bool lead, bool v
   }
 }
 
-// Sum the staged rows of the block's 32 cells in cell order: value entry q goes to row
-// vbase + vstep * q of out, tangent entry q to row tbase + tstep * q (rows of length stride).
+// Sum the staged rows of the block's 32 cells: value entry q goes to row vbase + vstep * q of
+// out, tangent entry q to row tbase + tstep * q (rows of length stride). Thread t sums cells
+// 8p..8p+7 of one (entry, half), p = t mod 4, and the four parts are added by two shuffles
+// (a fixed order). Entry `extra` (< 0: none), if staged, goes to row xbase / xbase + 1.
 template <typename T>
 __device__ __forceinline__ void flush(const double* vsh, int nq, double* out, int64_t vbase,
                                       int64_t vstep, int64_t tbase, int64_t tstep,
-                                      int64_t stride) {
+                                      int64_t stride, int extra = -1, int64_t xbase = 0) {
   __syncthreads();
-  const int q = threadIdx.x & 31;
-  const int half = threadIdx.x >> 5;
-  if (threadIdx.x < 64 && q < nq) {
-    // without dual numbers the tangent rows are written as zeros
-    double s = 0.0;
-    if (half == 0 || IsDual<T>::value)
-      for (int c = 0; c < kAdjCells; ++c) s += vsh[c * 64 + half * 32 + q];
-    const int64_t r = half == 0 ? vbase + vstep * q : tbase + tstep * q;
+  const int part = threadIdx.x & 3;
+  const int col = threadIdx.x >> 2;        // 0..63: half * 32 + q
+  const int q = col & 31, half = col >> 5;
+  double s = 0.0;
+  // without dual numbers the tangent rows are written as zeros
+  if ((q < nq || q == extra) && (half == 0 || IsDual<T>::value)) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += vsh[(part * 8 + c) * 64 + col];
+  }
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (part == 0 && (q < nq || q == extra)) {
+    int64_t r;
+    if (q == extra) r = xbase + half;
+    else r = half == 0 ? vbase + vstep * q : tbase + tstep * q;
     out[r * stride + blockIdx.x] = s;
   }
   __syncthreads();
@@ -333,9 +340,6 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
     stage(vrow, 31, ps, lead, valid);
     flush<T>(vsh, kNps, ppart, (int64_t)2 * s * kNps, 2, (int64_t)2 * s * kNps + 1, 2, nblk);
   }
-  stage(vrow, 0, dkt, lead, valid);
-  flush<T>(vsh, 1, ppart, (int64_t)2 * max_states * kNps, 0, (int64_t)2 * max_states * kNps + 1,
-           0, nblk);
 
   // vertex gradient: corner b is the upper end of the d edges of corners b and b ^ (1 << d)
   // when bit d of b is set, the lower end otherwise (averaged model: a quarter of the row)
@@ -374,7 +378,26 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
     for (int j = 0; j < 3; ++j)
       stage(vrow, 10 + 3 * d + j, sum8(moves ? g[j] : make<T>(0.0, 0.0)), lead, valid);
   }
-  flush<T>(vsh, kSysQ, part, 0, 1, kSysQ, 1, nblk);
+  // dE/dkT rides along in column kSysQ of the same flush, into its parameter rows
+  stage(vrow, kSysQ, dkt, lead, valid);
+  __syncthreads();
+  {
+    // the system rows go to `part`, the kT entry to `ppart`: two writers of one staged block
+    const int pq = threadIdx.x & 3;
+    const int col = threadIdx.x >> 2;
+    const int q = col & 31, half = col >> 5;
+    double sum = 0.0;
+    if (q <= kSysQ && (half == 0 || IsDual<T>::value)) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) sum += vsh[(pq * 8 + c) * 64 + col];
+    }
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    if (pq == 0 && q < kSysQ)
+      part[(int64_t)(half == 0 ? q : kSysQ + q) * nblk + blockIdx.x] = sum;
+    if (pq == 0 && q == kSysQ)
+      ppart[((int64_t)2 * max_states * kNps + half) * nblk + blockIdx.x] = sum;
+  }
 }
 
 // Row sums of per-block partials over lists of segments (8 blocks each): one block per
