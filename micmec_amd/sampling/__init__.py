@@ -6,12 +6,12 @@ from .iterative import (Iterative, StateItem, AttributeStateItem, PosStateItem,
 from .verlet import VerletIntegrator, VerletHook, VerletScreenLog, ConsErrTracker
 from .nvt import NHChain, NHCThermostat, NHCAttributeStateItem
 from .npt import TBCombination, MTKBarostat, MTKAttributeStateItem
-from .dof import DOF, CartesianDOF
+from .dof import DOF, CartesianDOF, BaseCellDOF, FullCellDOF, StrainCellDOF
 from .opt import OptScreenLog, BaseOptimizer, CGOptimizer, ConjugateGradientSearch
 
 __all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
            "TemperatureStateItem", "VolumeStateItem", "DomainStateItem", "ConsErrStateItem",
            "EPotContribStateItem", "Hook", "VerletIntegrator", "VerletHook", "VerletScreenLog",
            "ConsErrTracker", "NHChain", "NHCThermostat", "NHCAttributeStateItem",
-           "TBCombination", "MTKBarostat", "MTKAttributeStateItem", "DOF", "CartesianDOF",
+           "TBCombination", "MTKBarostat", "MTKAttributeStateItem", "DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF", "StrainCellDOF",
            "OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch"]

@@ -13,7 +13,7 @@ the end of a run (``sync_from_device``), not after every iteration.
 import numpy as np
 import torch
 
-__all__ = ["DOF", "CartesianDOF"]
+__all__ = ["DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF", "StrainCellDOF"]
 
 
 class _DeviceEnergy(object):
@@ -32,18 +32,29 @@ class _DeviceEnergy(object):
         kw = dict(dtype=torch.float64, device=self.device)
         self.energy = torch.zeros(1, **kw)
         self.gpos = torch.zeros((mmf.system.nnodes, 3), **kw)
+        self.vtens = torch.zeros((3, 3), **kw)
         self.rvecs = self.fpm.rvecs_device()
         self.calls = 0
 
-    def __call__(self, pos, do_gradient=True):
+    def __call__(self, pos, do_gradient=True, with_vtens=False):
+        """Energy and gradient at device positions ``pos`` and the system's current domain
+        (refresh ``self.rvecs`` with ``fpm.rvecs_device()`` after the domain changes); with
+        ``with_vtens`` also the virial (host 3x3 array), pressure part included."""
         self.calls += 1
-        self.fpm.compute_device(pos, self.rvecs, self.energy, self.gpos if do_gradient else None)
+        self.fpm.compute_device(pos, self.rvecs, self.energy, self.gpos if do_gradient else None,
+                                self.vtens if with_vtens else None)
         e = float(self.energy.item())
         if np.isnan(e):
             raise ValueError("The energy is not-a-number (``nan``).")
+        volume = self.fpm.system.domain.volume
         if self.press is not None:
-            e += self.press.pext * self.fpm.system.domain.volume
-        return e, self.gpos
+            e += self.press.pext * volume
+        if not with_vtens:
+            return e, self.gpos
+        vt = self.vtens.cpu().numpy()
+        if self.press is not None:
+            vt = vt + self.press.pext * volume * np.identity(3)
+        return e, self.gpos, vt
 
 
 class DOF(object):
@@ -145,6 +156,19 @@ class CartesianDOF(DOF):
             return v, self._gx.copy()
         return self.mmf.compute()
 
+    def _accept(self, x, g):
+        """Make the accepted point x (with gradient g) of an optimiser the current state."""
+        if isinstance(x, torch.Tensor):
+            if self.select is None:
+                self._pos_dev = x.view(-1, 3)
+                self._gpos_dev = g.view(-1, 3)
+        else:
+            self._update(x)
+            if self.select is None:
+                self._gpos[:] = np.asarray(g).reshape(-1, 3)
+            else:
+                self._gpos[self.select] = np.asarray(g).reshape(-1, 3)
+
     def sync_from_device(self):
         """Copy the last device positions and gradient into the host arrays and the system."""
         if getattr(self, "_pos_dev", None) is not None:
@@ -221,3 +245,301 @@ class CartesianDOF(DOF):
         self.conv_val, self.conv_worst = max(conv_vals)
         self.conv_count = sum(int(v >= 1) for v, n in conv_vals)
         self.converged = self.conv_count == 0
+
+
+class BaseCellDOF(DOF):
+    """Cell variables and fractional coordinates (dof.py:199-484).
+
+    x = [celldofs, frac] (frac omitted with ``do_frozen``): ``celldofs`` is the ``freemask``
+    selection of the subclass's cell variables, ``frac`` the fractional node coordinates in the
+    full domain matrix. The gradient with respect to the cell variables is
+    (gvecs . vtens) : d rvecs / d celldofs, with respect to frac gpos . rvecs^T; ``grvecs`` for the
+    convergence test is gvecs . vtens projected on the span of the Jacobian. Device tensors x are
+    evaluated with the HIP kernels: fractional coordinates, positions and gradients stay on the
+    GPU, only the cell variables (at most 9 numbers) and the virial cross to the host.
+    """
+
+    def __init__(self, mmf, gpos_rms=1e-5, dpos_rms=1e-3, grvecs_rms=1e-5, drvecs_rms=1e-3,
+                 do_frozen=False, freemask=None):
+        if freemask is not None:
+            if not (isinstance(freemask, np.ndarray) and issubclass(freemask.dtype.type, np.bool_)
+                    and len(freemask.shape) == 1 and freemask.sum() > 0):
+                raise TypeError("When given, freemask must be a vector of booleans.")
+        self.th_gpos_rms = gpos_rms
+        self.th_dpos_rms = dpos_rms
+        self.th_grvecs_rms = grvecs_rms
+        self.th_drvecs_rms = drvecs_rms
+        self.do_frozen = do_frozen
+        self.freemask = freemask
+        DOF.__init__(self, mmf)
+        self._last_pos = None
+        self._last_rvecs = None
+        self._last_pos_dev = None
+        self._pos_dev = None
+        self._gpos_dev = None
+        self._eval_x = None
+        try:
+            self._device = _DeviceEnergy(mmf)
+        except NotImplementedError:
+            self._device = None
+
+    @property
+    def device(self):
+        return None if self._device is None else self._device.device
+
+    ncellvar = property(lambda self: len(self.domainvars0))
+
+    @property
+    def ncelldof(self):
+        return len(self.domainvars0) if self.freemask is None else int(self.freemask.sum())
+
+    def _reduce_cellvars(self, cellvars):
+        return cellvars if self.freemask is None else cellvars[self.freemask]
+
+    def _expand_celldofs(self, celldofs):
+        if self.freemask is None:
+            return celldofs
+        cellvars = self.domainvars0.copy()
+        cellvars[self.freemask] = celldofs
+        return cellvars
+
+    def _isfree(self, icellvar):
+        return True if self.freemask is None else bool(self.freemask[icellvar])
+
+    def _init_initial(self):
+        self.domainvars0 = self._get_initial_cellvars()
+        if self.freemask is not None and len(self.freemask) != self.ncellvar:
+            raise TypeError("The length of the freemask vector (%i) does not match the number "
+                            "of cellvars (%i)." % (len(self.freemask), len(self.domainvars0)))
+        celldofs0 = self._reduce_cellvars(self.domainvars0)
+        domain = self.mmf.system.domain
+        frac = np.dot(np.asarray(self.mmf.system.pos), domain._get_gvecs(full=True).T)
+        if self.do_frozen:
+            self.x0 = celldofs0.copy()
+            self._frac0 = frac
+        else:
+            self.x0 = np.concatenate([celldofs0, frac.ravel()])
+        pos = np.asarray(self.mmf.system.pos)
+        self._pos = pos.copy()
+        self._dpos = np.zeros(pos.shape, float)
+        self._gpos = np.zeros(pos.shape, float)
+        self._rvecs = np.array(domain.rvecs, dtype=float)
+        self._dcell = np.zeros(self._rvecs.shape, float)
+        self._vtens = np.zeros((3, 3), float)
+        self._grvecs = np.zeros(self._rvecs.shape, float)
+
+    # -------------------------------------------------------------------------- evaluation
+    def _set_cell(self, celldofs):
+        """Domain vectors of the cell variables, stored in the force field (host)."""
+        self._rvecs = self._cellvars_to_rvecs(self._expand_celldofs(celldofs))
+        self.mmf.update_rvecs(self._rvecs[:])
+        return self.mmf.system.domain._get_rvecs(full=True)
+
+    def _cell_gradient(self, celldofs, vtens):
+        """Cell part of the gradient, and grvecs projected on the Jacobian's span."""
+        grvecs = np.dot(self.mmf.system.domain.gvecs, vtens)
+        jacobian = self._get_celldofs_jacobian(celldofs)
+        assert jacobian.shape == (grvecs.size, self.ncelldof)
+        gcell = np.dot(grvecs.ravel(), jacobian)
+        U, S, Vt = np.linalg.svd(jacobian, full_matrices=False)
+        self._grvecs[:] = np.dot(U, np.dot(U.T, grvecs.ravel())).reshape(-1, 3)
+        return gcell
+
+    def _update(self, x):
+        ncd = self.ncelldof
+        rvecs_full = self._set_cell(x[:ncd])
+        frac = self._frac0 if self.do_frozen else x[ncd:].reshape(-1, 3)
+        self._pos[:] = np.dot(frac, rvecs_full)
+        self.mmf.update_pos(self._pos[:])
+
+    def fun(self, x, do_gradient=False):
+        """Energy and, with do_gradient, its gradient with respect to x (dof.py:360-392)."""
+        ncd = self.ncelldof
+        if isinstance(x, torch.Tensor) and x.is_cuda:
+            if self._device is None:
+                raise NotImplementedError("device evaluation needs one ForcePartMechanical")
+            celldofs = x[:ncd].detach().cpu().numpy()
+            rvecs_full = self._set_cell(celldofs)
+            self._device.rvecs = self._device.fpm.rvecs_device()
+            R = torch.tensor(np.array(rvecs_full, dtype=float), device=x.device)
+            frac = (torch.as_tensor(self._frac0, dtype=torch.float64, device=x.device)
+                    if self.do_frozen else x[ncd:].view(-1, 3))
+            pos = torch.matmul(frac, R).contiguous()
+            self._pos_dev = pos
+            self._eval_x = x
+            if not do_gradient:
+                return self._device(pos, False)[0]
+            e, g, vtens = self._device(pos, True, with_vtens=True)
+            self._gpos_dev = g.clone()
+            self._vtens[:] = vtens
+            gx = torch.empty_like(x)
+            gx[:ncd] = torch.as_tensor(self._cell_gradient(celldofs, vtens), device=x.device)
+            if not self.do_frozen:
+                gx[ncd:] = torch.matmul(self._gpos_dev, R.t()).reshape(-1)
+            return e, gx
+        self._update(x)
+        self._eval_x = x
+        if do_gradient:
+            self._gpos[:] = 0.0
+            self._vtens[:] = 0.0
+            v = self.mmf.compute(self._gpos, self._vtens)
+            self._gx[:ncd] = self._cell_gradient(x[:ncd], self._vtens)
+            if not self.do_frozen:
+                self._gx[ncd:] = np.dot(self._gpos, self._rvecs.T).ravel()
+            return v, self._gx.copy()
+        return self.mmf.compute()
+
+    def _accept(self, x, g):
+        """Make the accepted point x of an optimiser the current state (re-evaluated unless it
+        was the last point evaluated)."""
+        if x is not self._eval_x:
+            self.fun(x, True)
+
+    def sync_from_device(self):
+        if self._pos_dev is not None:
+            self._pos[:] = self._pos_dev.cpu().numpy()
+            self.mmf.update_pos(self._pos[:])
+        if self._gpos_dev is not None:
+            self._gpos[:] = self._gpos_dev.cpu().numpy()
+
+    # -------------------------------------------------------------------------- convergence
+    def check_convergence(self):
+        """Convergence criteria of dof.py:394-457 (positions and gradients reduced on the GPU on
+        the device path)."""
+        device = self._pos_dev is not None and self._gpos_dev is not None
+        if self._last_rvecs is None:
+            self._last_rvecs = self._rvecs.copy()
+            if device:
+                self._last_pos_dev = self._pos_dev.detach().clone()
+            else:
+                self._last_pos = self._pos.copy()
+            self.converged = False
+            self.conv_val = 2
+            self.conv_worst = "first_step"
+            self.conv_count = -1
+            return
+        if device:
+            g, d = self._gpos_dev, self._pos_dev - self._last_pos_dev
+            gsq, dsq = (g * g).sum(dim=1), (d * d).sum(dim=1)
+            m = torch.stack([gsq.max(), gsq.mean(), dsq.max(), dsq.mean(),
+                             gsq.argmax().to(torch.float64)]).cpu().numpy()
+            gmax, grms = np.sqrt(m[0]), np.sqrt(m[1])
+            self.dpos_max, self.dpos_rms = np.sqrt(m[2]), np.sqrt(m[3])
+            gind = int(m[4])
+            self._last_pos_dev = self._pos_dev.detach().clone()
+        else:
+            gsq = (self._gpos ** 2).sum(axis=1)
+            gmax, grms, gind = np.sqrt(gsq.max()), np.sqrt(gsq.mean()), int(gsq.argmax())
+            self._dpos[:] = self._pos - self._last_pos
+            dsq = (self._dpos ** 2).sum(axis=1)
+            self.dpos_max, self.dpos_rms = np.sqrt(dsq.max()), np.sqrt(dsq.mean())
+            self._last_pos[:] = self._pos
+        if not self.do_frozen:
+            self.gpos_max, self.gpos_rms, self.gpos_indmax = gmax, grms, gind
+        grsq = (self._grvecs ** 2).sum(axis=1)
+        self.grvecs_max = np.sqrt(grsq.max())
+        self.grvecs_rms = np.sqrt(grsq.mean())
+        self._dcell[:] = self._rvecs - self._last_rvecs
+        dcsq = (self._dcell ** 2).sum(axis=1)
+        self.drvecs_max = np.sqrt(dcsq.max())
+        self.drvecs_rms = np.sqrt(dcsq.mean())
+        conv_vals = []
+        if not self.do_frozen and self.th_gpos_rms is not None:
+            conv_vals.append((self.gpos_rms / self.th_gpos_rms, "gpos_rms"))
+            conv_vals.append((self.gpos_max / (self.th_gpos_rms * 3),
+                              "gpos_max(%i)" % self.gpos_indmax))
+        if self.th_dpos_rms is not None:
+            conv_vals.append((self.dpos_rms / self.th_dpos_rms, "dpos_rms"))
+            conv_vals.append((self.dpos_max / (self.th_dpos_rms * 3), "dpos_max"))
+        if self.th_grvecs_rms is not None:
+            conv_vals.append((self.grvecs_rms / self.th_grvecs_rms, "grvecs_rms"))
+            conv_vals.append((self.grvecs_max / (self.th_grvecs_rms * 3), "grvecs_max"))
+        if self.th_drvecs_rms is not None:
+            conv_vals.append((self.drvecs_rms / self.th_drvecs_rms, "drvecs_rms"))
+            conv_vals.append((self.drvecs_max / (self.th_drvecs_rms * 3), "drvecs_max"))
+        if len(conv_vals) == 0:
+            raise RuntimeError("At least one convergence criterion must be present.")
+        self.conv_val, self.conv_worst = max(conv_vals)
+        self.conv_count = sum(int(v >= 1) for v, n in conv_vals)
+        self.converged = self.conv_count == 0
+        self._last_rvecs[:] = self._rvecs
+        if self.converged and device:
+            self.sync_from_device()
+
+    # -------------------------------------------------------------------------- subclass API
+    def _get_initial_cellvars(self):
+        raise NotImplementedError
+
+    def _cellvars_to_rvecs(self, cellvars):
+        raise NotImplementedError
+
+    def _get_celldofs_jacobian(self, x):
+        raise NotImplementedError
+
+
+class FullCellDOF(BaseCellDOF):
+    """All components of the domain vectors, divided by s = V^(1/nvec) of the initial domain
+    so that the cell variables are dimensionless (dof.py:487-514)."""
+
+    def _get_initial_cellvars(self):
+        domain = self.mmf.system.domain
+        if domain.nvec == 0:
+            raise ValueError("A cell optimization requires a system that is periodic.")
+        self._rvecs_scale = domain.volume ** (1.0 / domain.nvec)
+        return np.array(domain.rvecs, dtype=float).ravel() / self._rvecs_scale
+
+    def _cellvars_to_rvecs(self, cellvars):
+        return np.asarray(cellvars, dtype=float).reshape(-1, 3) * self._rvecs_scale
+
+    def _get_celldofs_jacobian(self, x):
+        result = np.identity(self.ncellvar) * self._rvecs_scale
+        if self.freemask is not None:
+            result = result[:, self.freemask]
+        return result
+
+
+class StrainCellDOF(BaseCellDOF):
+    """A symmetric transformation A of the initial domain vectors, rotations eliminated
+    (dof.py:517-697). Cell variables: 3D [A00, A11, A22, 2 A12, 2 A20, 2 A01],
+    2D [A00, A11, 2 A01], 1D [A00]."""
+
+    # (row, column) pairs of the symmetric matrix and their weight per cell variable
+    _PAIRS = {3: [((0, 0),), ((1, 1),), ((2, 2),), ((1, 2), (2, 1)), ((2, 0), (0, 2)),
+                  ((0, 1), (1, 0))],
+              2: [((0, 0),), ((1, 1),), ((0, 1), (1, 0))],
+              1: [((0, 0),)]}
+
+    def _get_initial_cellvars(self):
+        domain = self.mmf.system.domain
+        if domain.nvec == 0:
+            raise ValueError("A cell optimization requires a system that is periodic.")
+        self.rvecs0 = np.array(domain.rvecs, dtype=float)
+        nvec = domain.nvec
+        return np.array([1.0] * nvec + [0.0] * (nvec * (nvec + 1) // 2 - nvec))
+
+    def _deform(self, cellvars):
+        nvec = len(self.rvecs0)
+        A = np.zeros((nvec, nvec))
+        for k, pairs in enumerate(self._PAIRS[nvec]):
+            w = 1.0 if len(pairs) == 1 else 0.5
+            for (i, j) in pairs:
+                A[i, j] = w * cellvars[k]
+        return A
+
+    def _cellvars_to_rvecs(self, cellvars):
+        nvec = len(self.rvecs0)
+        return np.dot(self._deform(np.asarray(cellvars)[: nvec * (nvec + 1) // 2]), self.rvecs0)
+
+    def _get_celldofs_jacobian(self, x):
+        # d rvecs[i, :] / d cellvar k = sum over the (i, j) pairs of cellvar k of w * rvecs0[j]
+        nvec = len(self.rvecs0)
+        cols = []
+        for k, pairs in enumerate(self._PAIRS[nvec]):
+            if not self._isfree(k):
+                continue
+            w = 1.0 if len(pairs) == 1 else 0.5
+            col = np.zeros((nvec, 3))
+            for (i, j) in pairs:
+                col[i] += w * self.rvecs0[j]
+            cols.append(col.ravel())
+        return np.array(cols).T

@@ -123,30 +123,39 @@ class ConjugateGradientSearch(object):
                 return True
         f0 = self.f
         ftol = 8.0 * np.finfo(float).eps * max(abs(f0), np.finfo(float).tiny)
-        a_lo, dp_lo = 0.0, dphi0
+        # bracket [lo, hi] of the minimum along d: lo has phi'(lo) < 0 and phi(lo) <= phi(0),
+        # hi has phi'(hi) > 0 or no decrease; the next trial is the secant zero of phi' in the
+        # bracket (exact on quadratics), kept a tenth of the width away from its ends
+        lo_a, lo_dp = 0.0, dphi0
+        hi_a, hi_dp = None, None
         a = self.alpha
         best = None
         for _ in range(self.max_ls):
             xa = self.x + a * self.d
             fa, ga = self.fun(xa, True)
             dp = _dot(ga, self.d)
-            ok = np.isfinite(fa) and fa <= f0 + ftol
+            ok = np.isfinite(fa) and np.isfinite(dp) and fa <= f0 + ftol
             if ok and (best is None or abs(dp) < abs(best[4])):
                 best = (a, fa, ga, xa, dp)
             if ok and abs(dp) <= self.curvature * abs(dphi0):
                 break
-            # secant step on phi'(alpha) through the two latest points; exact on quadratics
-            if np.isfinite(fa) and dp > dp_lo:
-                a_new = a - dp * (a - a_lo) / (dp - dp_lo)
-            elif np.isfinite(fa) and dp < 0.0:
-                a_new = 2.0 * a
+            if ok and dp < 0.0:
+                lo_a, lo_dp = a, dp
             else:
-                a_new = 0.5 * (a + a_lo)
-            if not np.isfinite(a_new) or a_new <= 0.0:
-                a_new = 0.5 * a
+                hi_a, hi_dp = a, (dp if ok or (np.isfinite(fa) and np.isfinite(dp)) else None)
+            if hi_a is None:
+                a_new = 4.0 * a
+            else:
+                if hi_dp is not None and hi_dp > 0.0:
+                    a_new = lo_a - lo_dp * (hi_a - lo_a) / (hi_dp - lo_dp)
+                else:
+                    a_new = 0.5 * (lo_a + hi_a)
+                width = hi_a - lo_a
+                if not np.isfinite(a_new):
+                    a_new = lo_a + 0.5 * width
+                a_new = min(max(a_new, lo_a + 0.1 * width), hi_a - 0.1 * width)
             if self.max_step is not None:
                 a_new = min(a_new, self.max_step)
-            a_lo, dp_lo = a, dp
             a = a_new
         if best is None:
             return False
@@ -181,14 +190,7 @@ class CGOptimizer(BaseOptimizer):
         self.x = self.minimizer.x
         # make the accepted point the current state of the degrees of freedom
         self.epot = self.minimizer.f
-        if isinstance(self.x, torch.Tensor):
-            self.dof._pos_dev = self.x.view(-1, 3) if self.dof.select is None else \
-                self.dof._pos_dev
-            self.dof._gpos_dev = self.minimizer.g.view(-1, 3) if self.dof.select is None else \
-                self.dof._gpos_dev
-        else:
-            self.dof._update(self.x)
-            self.dof._gpos[:] = self.minimizer.g.reshape(-1, 3)
+        self.dof._accept(self.x, self.minimizer.g)
         if not success:
             log.warn("Line search failed in optimizer. Aborting optimization.")
             return True

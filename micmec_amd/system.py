@@ -65,17 +65,26 @@ class Domain(object):
     def volume(self):
         return self._volume
 
-    @property
-    def rvecs(self):
-        out = self._rvecs_full[: self._nvec].copy()
+    def _get_rvecs(self, full=False):
+        """Domain vectors; with ``full`` the 3x3 matrix completed by orthonormal directions
+        (ext.pyx:84-94)."""
+        out = (self._rvecs_full if full else self._rvecs_full[: self._nvec]).copy()
+        out.setflags(write=False)
+        return out
+
+    def _get_gvecs(self, full=False):
+        """Reciprocal vectors (ext.pyx:96-106)."""
+        out = (self._gvecs_full if full else self._gvecs_full[: self._nvec]).copy()
         out.setflags(write=False)
         return out
 
     @property
+    def rvecs(self):
+        return self._get_rvecs()
+
+    @property
     def gvecs(self):
-        out = self._gvecs_full[: self._nvec].copy()
-        out.setflags(write=False)
-        return out
+        return self._get_gvecs()
 
     @property
     def parameters(self):

@@ -32,14 +32,32 @@ def test_cg_search_minimises_a_quadratic():
     assert np.allclose(cg.x, np.linalg.solve(A, b), atol=1e-9)
 
 
-def test_cg_search_on_rosenbrock():
+@pytest.mark.parametrize("x0", [[-1.2, 1.0], [1.3, 0.7, 0.8, 1.9, 1.2]])
+def test_cg_search_on_rosenbrock(x0):
+    # the textbook starts; both lie in the basin of the global minimum at x = 1
     f = lambda x, g=True: (scipy.optimize.rosen(x), scipy.optimize.rosen_der(x))
-    cg = ConjugateGradientSearch(np.array([-1.2, 1.0, 0.5, -0.3]), f)
+    cg = ConjugateGradientSearch(np.array(x0), f)
     cg.initialize()
     for _ in range(2000):
         if not cg.propagate() or np.linalg.norm(cg.g) < 1e-10:
             break
     assert np.allclose(cg.x, 1.0, atol=1e-5)
+
+
+def test_cg_search_line_search_recovers_from_an_overshooting_first_step():
+    # a stiff direction (curvature 1e6) next to a soft one: the first trial step 1/|g| overshoots
+    # the minimum along the steepest descent by orders of magnitude, and the line search has to
+    # bracket back instead of giving up
+    k = np.array([1e6, 1.0, 3.0])
+    f = lambda x, g=True: (0.5 * np.dot(k * x, x), k * x)
+    cg = ConjugateGradientSearch(np.array([1e-3, 2.0, -1.0]), f)
+    cg.initialize()
+    assert cg.propagate()
+    assert cg.f < 0.5 * np.dot(k * np.array([1e-3, 2.0, -1.0]), [1e-3, 2.0, -1.0])
+    for _ in range(50):
+        if not cg.propagate() or np.linalg.norm(cg.g) < 1e-12:
+            break
+    assert np.linalg.norm(cg.x) < 1e-9
 
 
 @pytest.mark.gpu
