@@ -125,7 +125,7 @@ class ConjugateGradientSearch(object):
         ftol = 8.0 * np.finfo(float).eps * max(abs(f0), np.finfo(float).tiny)
         # bracket [lo, hi] of the minimum along d: lo has phi'(lo) < 0 and phi(lo) <= phi(0),
         # hi has phi'(hi) > 0 or no decrease; the next trial is the secant zero of phi' in the
-        # bracket (exact on quadratics), kept a tenth of the width away from its ends
+        # bracket (exact on quadratics), kept a hundredth of the width away from its ends
         lo_a, lo_dp = 0.0, dphi0
         hi_a, hi_dp = None, None
         a = self.alpha
@@ -140,11 +140,16 @@ class ConjugateGradientSearch(object):
             if ok and abs(dp) <= self.curvature * abs(dphi0):
                 break
             if ok and dp < 0.0:
+                prev_a, prev_dp = lo_a, lo_dp
                 lo_a, lo_dp = a, dp
             else:
                 hi_a, hi_dp = a, (dp if ok or (np.isfinite(fa) and np.isfinite(dp)) else None)
             if hi_a is None:
+                # still descending: extrapolate the secant of phi' through the last two points
+                # (beyond a when phi' rises), at most ten times as far
                 a_new = 4.0 * a
+                if dp > prev_dp:
+                    a_new = min(a - dp * (a - prev_a) / (dp - prev_dp), 10.0 * a)
             else:
                 if hi_dp is not None and hi_dp > 0.0:
                     a_new = lo_a - lo_dp * (hi_a - lo_a) / (hi_dp - lo_dp)
@@ -153,7 +158,7 @@ class ConjugateGradientSearch(object):
                 width = hi_a - lo_a
                 if not np.isfinite(a_new):
                     a_new = lo_a + 0.5 * width
-                a_new = min(max(a_new, lo_a + 0.1 * width), hi_a - 0.1 * width)
+                a_new = min(max(a_new, lo_a + 0.01 * width), hi_a - 0.01 * width)
             if self.max_step is not None:
                 a_new = min(a_new, self.max_step)
             a = a_new
