@@ -155,15 +155,17 @@ void mm_adjoint_destroy(mm_adjoint* a);
 
 #define MM_THERMO_NONE 0
 #define MM_THERMO_NHC 1
+#define MM_THERMO_LANGEVIN 2 /* LangevinThermostat (micmec/sampling/nvt.py:165-218) */
 #define MM_BARO_NONE 0
 #define MM_BARO_MTK 1
+#define MM_BARO_LANGEVIN 2 /* LangevinBarostat (micmec/sampling/npt.py:295-510) */
 
 /* Scalar block layout (doubles). Integer-valued entries are stored as doubles. */
 #define MM_S_TIMESTEP 0
 #define MM_S_NDOF 1
 #define MM_S_BOLTZMANN 2
-#define MM_S_THERMO 3
-#define MM_S_BARO 4
+#define MM_S_THERMO 3 /* MM_THERMO_* of the hook */
+#define MM_S_BARO 4   /* MM_BARO_* of the hook */
 #define MM_S_CHAIN_LEN 5
 #define MM_S_THERMO_TEMP 6
 #define MM_S_BARO_TEMP 7
@@ -172,6 +174,7 @@ void mm_adjoint_destroy(mm_adjoint* a);
 #define MM_S_BARO_NDOF 10
 #define MM_S_EXT_PRESS 11
 #define MM_S_NVEC 12
+#define MM_S_CHAIN_NDOF 13 /* degrees of freedom of the NHC chain (nvt.py:395-401) */
 #define MM_S_CHAIN_MASS 16 /* 8 */
 #define MM_S_CHAIN_POS 24  /* 8 */
 #define MM_S_CHAIN_VEL 32  /* 8 */
@@ -208,7 +211,20 @@ void mm_adjoint_destroy(mm_adjoint* a);
 #define MM_S_VSCALE_B 130 /* pending velocity factor applied last (thermostat pre) */
 #define MM_S_VSCALE_B_ON 131
 #define MM_S_NNODES 132 /* nodes of the system (mean of the rmsd values) */
-#define MM_S_COUNT 144
+/* Langevin hooks. Their Gaussian noise is Philox4x64-10 with the key (seed, stream) and the
+ * counter (node or system, 2 * counter + post, update, block); the seeds are 64-bit words held
+ * as two doubles with their 32-bit halves. */
+#define MM_S_LT_C1 133     /* exp(-dt / timecon / 2) */
+#define MM_S_LT_A 134      /* (1 - c1^2) T k_B; the noise amplitude of node n is sqrt(A / m_n) */
+#define MM_S_LT_SEED 135   /* 2 */
+#define MM_S_LB_SEED 137   /* 2 */
+#define MM_S_LB_TIMECON 139
+#define MM_S_LB_RSCALE 140 /* sqrt(2 W k_B T / (dt timecon)), the scale of the random force */
+#define MM_S_BARO_ANISO 141   /* 1: 3x3 barostat velocity, 0: isotropic (VEL_PRESS[0]) */
+#define MM_S_BARO_VOLCON 142  /* 1: volume-preserving cell fluctuations */
+#define MM_S_BARO_EPOT0 143   /* epot and ekin at the start of the barostat call */
+#define MM_S_BARO_EKIN0 144
+#define MM_S_COUNT 160
 
 typedef struct mm_md mm_md;
 

@@ -177,8 +177,9 @@ class BatchVerlet(object):
                 cm[0] *= ndof
                 cv = np.random.normal(0, np.sqrt(cm * units.boltzmann * self.temps[k]),
                                       chainlength) / cm
-                Sk[L.S_THERMO] = 1.0
+                Sk[L.S_THERMO] = L.THERMO_NHC
                 Sk[L.S_CHAIN_LEN] = chainlength
+                Sk[L.S_CHAIN_NDOF] = ndof
                 Sk[L.S_THERMO_TEMP] = self.temps[k]
                 Sk[L.S_CHAIN_MASS:L.S_CHAIN_MASS + chainlength] = cm
                 Sk[L.S_CHAIN_VEL:L.S_CHAIN_VEL + chainlength] = cv

@@ -1,4 +1,5 @@
-// Device MD engine: velocity Verlet with Nose-Hoover chain and MTK barostat (FP64).
+// Device MD engine: velocity Verlet with Nose-Hoover chain or Langevin thermostat and MTK or
+// Langevin barostat (FP64).
 //
 // Reference sequence (one step, micmec/sampling/verlet.py:140-166):
 //   pre hooks  -> v += a dt/2 -> x += v dt -> force -> v += a dt/2 -> ekin -> post hooks
@@ -10,6 +11,9 @@
 // the scalar block S, between the node kernels, so a whole step can be captured in a HIP graph.
 // Velocity/position transforms produced by a scalar phase (thermostat factor, barostat
 // rotation) are stored in S and applied by the next node pass that touches the array.
+// The Langevin thermostat's noise is drawn inside the node passes that already touch the
+// velocities (pre: the kick-drift pass, post: the gather-kick pass), from a counter-based
+// generator keyed by node and hook call, so it costs no extra pass over the nodes.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,17 +25,29 @@
 #include "mm_device.h"
 #include "mm_internal.h"
 #include "mm_kernels.h"
+#include "mm_rng.h"
 
 namespace mm {
 
 constexpr int kNodeBlock = 256;   // nodes per block (system boundaries are block aligned)
 constexpr int kNodeThreads = 64;  // one wave per block, kNodeBlock / kNodeThreads nodes a lane
-constexpr int kNodePartial = 16;  // [0] ekin [1..9] sum m v(x)v [10] sum g^2 [11] sum d^2 [12] nan
+// node sums: [0] ekin [1..9] sum m v(x)v [10] sum g^2 [11] sum d^2 [12] nan
+// [13] ekin before the Langevin post update [14] ekin after the Langevin pre update
+constexpr int kNodePartial = 16;
+constexpr int kNodeSums = 15;
 constexpr int kScalarThreads = 256;
 constexpr int kScalarSlices = 32;  // most blocks a scalar phase spreads its reduction over
-constexpr int kSliceQ = 23;        // force (10) and node (13) sums per slice
+constexpr int kSliceQ = 10 + kNodeSums;  // force and node sums per slice
 
-enum NodeMode { kKickDrift = 0, kScalePos = 1, kScaleVel = 2, kVV = 3, kApplyV = 4, kRmsd = 5 };
+enum NodeMode {
+  kKickDrift = 0,
+  kScalePos = 1,
+  kScaleVel = 2,
+  kVV = 3,
+  kApplyV = 4,
+  kRmsd = 5,
+  kKickDriftLT = 6,  // kick-drift preceded by the Langevin thermostat's pre update
+};
 
 enum Phase {
   kPhInitForce = 0,      // epot (+ vtens) after an initial force evaluation
@@ -80,7 +96,8 @@ __device__ __forceinline__ void block_sum_store_wide(double (&v)[NV], double* pa
 
 // Kinetic-energy and m v (x) v sums of one node, in the reference's operation order:
 // 0.5 * (v**2 * m) (verlet.py:169) and (v.T * m) . v (verlet.py:187-188, npt.py:661-663).
-__device__ __forceinline__ void node_vv(const double v[3], double m, double (&acc)[13]) {
+template <int NA>
+__device__ __forceinline__ void node_vv(const double v[3], double m, double (&acc)[NA]) {
 #pragma clang fp contract(off)
 #pragma unroll
   for (int j = 0; j < 3; ++j) acc[0] += 0.5 * ((v[j] * v[j]) * m);
@@ -88,6 +105,29 @@ __device__ __forceinline__ void node_vv(const double v[3], double m, double (&ac
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc[1 + 3 * i + j] += (v[i] * m) * v[j];
+}
+
+// LangevinThermostat.thermo (nvt.py:211-218): v = c1 v + sqrt((1 - c1^2) T k_B / m) xi with
+// xi the standard normals of node n at hook call 2 counter + post.
+__device__ __forceinline__ void langevin_thermo(const double* S, int64_t n, int post, double m,
+                                                double v[3]) {
+#pragma clang fp contract(off)
+  const uint64_t ctr[4] = {(uint64_t)n, 2ull * (uint64_t)S[MM_S_COUNTER] + (uint64_t)post, 0ull,
+                           0ull};
+  double z[4];
+  normals4(ctr, seed_from(S + MM_S_LT_SEED), kStreamThermostat, z);
+  const double c1 = S[MM_S_LT_C1];
+  const double c2 = sqrt(S[MM_S_LT_A] / m);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) v[j] = c1 * v[j] + c2 * z[j];
+}
+
+__device__ __forceinline__ double node_ekin(const double v[3], double m) {
+#pragma clang fp contract(off)
+  double e = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) e += 0.5 * ((v[j] * v[j]) * m);
+  return e;
 }
 
 // row vector times 3x3 matrix
@@ -130,17 +170,22 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
   double acc[13];
 #pragma unroll
   for (int q = 0; q < 13; ++q) acc[q] = 0.0;
+  double lt[1] = {0.0};
 #pragma unroll
   for (int it = 0; it < kNodeBlock / kNodeThreads; ++it) {
     const int64_t n = (int64_t)blockIdx.x * kNodeBlock + it * kNodeThreads + threadIdx.x;
     if (n >= N) break;
-    if (kMode == kKickDrift) {
+    if (kMode == kKickDrift || kMode == kKickDriftLT) {
       const double dt = S[MM_S_TIMESTEP];
       const double mass = m[n];
       double vv[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
       apply_vtransform(S, vv);
+      if (kMode == kKickDriftLT) {
+        langevin_thermo(S, n, 0, mass, vv);
+        lt[0] += node_ekin(vv, mass);
+      }
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double a = -g[3 * n + j] / mass;
@@ -178,21 +223,24 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
   if (kMode == kScaleVel || kMode == kVV || kMode == kRmsd) {
     block_sum_store<13>(acc, partials);
   }
+  if (kMode == kKickDriftLT) block_sum_store<1>(lt, partials + (int64_t)14 * gridDim.x);
 }
 
 // gpos[n] = sum_k gcell[k][:, w(n,k)] (mmff.py:303-318), fused with the second half kick
-// v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities.
+// v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities; with kLT also
+// the Langevin thermostat's post update (the node sums then hold the ekin before it in [13]).
 // One node per thread (the gathers need every wave in flight that they can get).
-template <bool kKick>
+template <bool kKick, bool kLT>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t W, double* __restrict__ gpos, double* __restrict__ v,
                    const double* __restrict__ m, const double* __restrict__ S0,
                    const int32_t* __restrict__ blk_sys, double* __restrict__ partials) {
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
-  double acc[13];
+  constexpr int NA = kLT ? 14 : 13;
+  double acc[NA];
 #pragma unroll
-  for (int q = 0; q < 13; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NA; ++q) acc[q] = 0.0;
   {
     const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
     if (n < N) {
@@ -221,14 +269,19 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       for (int j = 0; j < 3; ++j) {
         const double a = -g[j] / mass;
         vv[j] = v[3 * n + j] + (0.5 * a) * dt;
-        v[3 * n + j] = vv[j];
       }
+      if (kLT) {
+        acc[NA - 1] += node_ekin(vv, mass);
+        langevin_thermo(S, n, 1, mass, vv);
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
       node_vv(vv, mass, acc);
     }
     }
   }
   if (kKick) {
-    block_sum_store_wide<13>(acc, partials);
+    block_sum_store_wide<NA>(acc, partials);
   } else {
     // only the NaN flag: any lane's flag, per block
     const unsigned long long bad = __ballot(acc[12] != 0.0);
@@ -327,7 +380,7 @@ __device__ double nhc_chain(double* S, double* ekin_io, bool has_g1, double g1) 
   const double dt = S[MM_S_TIMESTEP];
   const double temp = S[MM_S_THERMO_TEMP];
   const double kb = S[MM_S_BOLTZMANN];
-  const double ndof = S[MM_S_NDOF];
+  const double ndof = S[MM_S_CHAIN_NDOF];
   double* mass = S + MM_S_CHAIN_MASS;
   double* pos = S + MM_S_CHAIN_POS;
   double* vel = S + MM_S_CHAIN_VEL;
@@ -367,7 +420,7 @@ __device__ double nhc_econs(const double* S) {
   for (int k = 0; k < L; ++k)
     kin += (S[MM_S_CHAIN_VEL + k] * S[MM_S_CHAIN_VEL + k]) * S[MM_S_CHAIN_MASS + k];
   for (int k = 1; k < L; ++k) rest += S[MM_S_CHAIN_POS + k];
-  return 0.5 * kin + kt * (S[MM_S_NDOF] * S[MM_S_CHAIN_POS] + rest);
+  return 0.5 * kin + kt * (S[MM_S_CHAIN_NDOF] * S[MM_S_CHAIN_POS] + rest);
 }
 
 // MTKBarostat._compute_ekin_baro (npt.py:754-757, anisotropic)
@@ -384,59 +437,158 @@ __device__ double add_press_cont(const double* S) {
   return 2.0 * baro_ekin(S) - S[MM_S_BARO_NDOF] * (S[MM_S_BARO_TEMP] * S[MM_S_BOLTZMANN]);
 }
 
-// update_baro_vel of MTKBarostat.baro (npt.py:654-681)
-__device__ void update_baro_vel(double* S) {
+__device__ __forceinline__ bool baro_langevin(const double* S) {
+  return S[MM_S_BARO] == (double)MM_BARO_LANGEVIN;
+}
+__device__ __forceinline__ bool baro_aniso(const double* S) {
+  return !baro_langevin(S) || S[MM_S_BARO_ANISO] != 0.0;
+}
+__device__ __forceinline__ bool thermo_nhc(const double* S) {
+  return S[MM_S_THERMO] == (double)MM_THERMO_NHC;
+}
+
+// LangevinBarostat.getR (npt.py:489-505): a symmetric matrix whose lower triangle is a draw of
+// N(0, 1) x RSCALE, here the normals of system sys at hook call 2 counter + post, update sub.
+__device__ void langevin_random_force(const double* S, int sys, int post, int sub, double R[9]) {
+#pragma clang fp contract(off)
+  const uint64_t call = 2ull * (uint64_t)S[MM_S_COUNTER] + (uint64_t)post;
+  const uint64_t key = seed_from(S + MM_S_LB_SEED);
+  double z[8];
+  for (int b = 0; b < 2; ++b) {
+    const uint64_t ctr[4] = {(uint64_t)sys, call, (uint64_t)sub, (uint64_t)b};
+    normals4(ctr, key, kStreamBarostat, z + 4 * b);
+  }
+  const double sc = S[MM_S_LB_RSCALE];
+  int k = 0;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      const double r = z[k++] * sc;
+      R[3 * i + j] = r;
+      R[3 * j + i] = r;
+    }
+  }
+}
+
+// update_baro_vel of MTKBarostat.baro (npt.py:654-681) and of LangevinBarostat.baro
+// (npt.py:394-432: friction exp(-dt / 8 tau) on both sides, random force R, optional volume
+// constraint and isotropic form). sub is 0 for the first and 1 for the second update of a call.
+__device__ void update_baro_vel(double* S, int sys, int post, int sub) {
 #pragma clang fp contract(off)
   const double dt = S[MM_S_TIMESTEP];
   double* vp = S + MM_S_VEL_PRESS;
+  const bool lb = baro_langevin(S);
+  const bool aniso = baro_aniso(S);
+  const int nv = aniso ? 9 : 1;
   const bool has = S[MM_S_HAS_CHAINVEL0] != 0.0;
   const double cv0 = S[MM_S_CHAINVEL0];
+  const double fric = lb ? exp((-dt) / (8.0 * S[MM_S_LB_TIMECON])) : 1.0;
+  if (lb)
+    for (int i = 0; i < nv; ++i) vp[i] *= fric;
   if (has) {
     const double f = exp(((-dt) * cv0) / 8.0);
-    for (int i = 0; i < 9; ++i) vp[i] *= f;
+    for (int i = 0; i < nv; ++i) vp[i] *= f;
   }
-  double P[9];
+  double P[9], G[9];
   for (int i = 0; i < 9; ++i) P[i] = S[MM_S_SVV + i] - S[MM_S_VTENS + i];
   const double diag = (2.0 * S[MM_S_EKIN]) / S[MM_S_NDOF] - S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) {
       double p = 0.5 * (P[3 * j + i] + P[3 * i + j]);
       if (i == j) p = p + diag;
-      const double G = p / S[MM_S_MASS_PRESS];
-      vp[3 * i + j] += (G * dt) / 4.0;
+      G[3 * i + j] = p / S[MM_S_MASS_PRESS];
     }
+  }
+  if (!lb) {
+    for (int i = 0; i < 9; ++i) vp[i] += (G[i] * dt) / 4.0;
+  } else {
+    const double W = S[MM_S_MASS_PRESS];
+    double R[9];
+    langevin_random_force(S, sys, post, sub, R);
+    if (S[MM_S_BARO_VOLCON] != 0.0) {
+      const double dim = S[MM_S_NVEC];
+      const double tg = ((G[0] + G[4]) + G[8]) / dim;
+      const double tr = ((R[0] + R[4]) + R[8]) / dim;
+      for (int i = 0; i < 9; i += 4) {
+        G[i] -= tg;
+        R[i] -= tr;
+      }
+    }
+    if (aniso) {
+      for (int i = 0; i < 9; ++i) vp[i] += ((G[i] - R[i] / W) * dt) / 4.0;
+    } else {
+      const double g = (G[0] + G[4]) + G[8];
+      vp[0] += ((g - R[0] / W) * dt) / 4.0;
+    }
+    for (int i = 0; i < nv; ++i) vp[i] *= fric;
   }
   if (has) {
     const double f = exp(((-dt) * cv0) / 8.0);
-    for (int i = 0; i < 9; ++i) vp[i] *= f;
+    for (int i = 0; i < nv; ++i) vp[i] *= f;
   }
 }
 
-// First half of MTKBarostat.baro: v_g update, then positions and domain scaled by
-// exp(v_g dt / 2) (npt.py:683-702). The position transform is left in XMAT.
-__device__ void baro_part1(double* S) {
-  update_baro_vel(S);
+// First half of the barostat call: v_g update, then positions and domain scaled by
+// exp(v_g dt / 2) (npt.py:683-702, 434-452). The position transform is left in XMAT.
+__device__ void baro_part1(double* S, int sys, int post) {
+#pragma clang fp contract(off)
+  update_baro_vel(S, sys, post, 0);
   double R[9];
-  expm_sym_half(S + MM_S_VEL_PRESS, 1.0, S[MM_S_TIMESTEP], R);
+  if (baro_aniso(S)) {
+    expm_sym_half(S + MM_S_VEL_PRESS, 1.0, S[MM_S_TIMESTEP], R);
+  } else {
+    const double c = exp((S[MM_S_VEL_PRESS] * S[MM_S_TIMESTEP]) / 2.0);
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? c : 0.0;
+  }
   for (int i = 0; i < 9; ++i) S[MM_S_XMAT + i] = R[i];
   matmul3(S + MM_S_RVECS, R, S + MM_S_RVECS);
   S[MM_S_VOLUME] = volume3(S + MM_S_RVECS);
 }
 
-// Velocity transform of MTKBarostat.baro: exp(-(v_g + tr(v_g)/ndof) dt / 2) (npt.py:709-726),
-// composed after any pending velocity factor.
+// Velocity transform of the barostat call: exp(-(v_g + tr(v_g)/ndof) dt / 2) (npt.py:709-726;
+// without the trace term under a volume constraint, and exp(-(1 + 3/ndof) v_g dt / 2) in the
+// isotropic form, npt.py:466-484), composed after any pending velocity factor.
 __device__ void baro_part2(double* S) {
 #pragma clang fp contract(off)
-  double A[9];
   const double* vp = S + MM_S_VEL_PRESS;
-  const double tr = (vp[0] + vp[4]) + vp[8];
-  for (int i = 0; i < 9; ++i) A[i] = vp[i];
-  const double sh = tr / S[MM_S_NDOF];
-  A[0] += sh;
-  A[4] += sh;
-  A[8] += sh;
-  expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], S + MM_S_VMAT);
+  if (baro_aniso(S)) {
+    double A[9];
+    for (int i = 0; i < 9; ++i) A[i] = vp[i];
+    if (!(baro_langevin(S) && S[MM_S_BARO_VOLCON] != 0.0)) {
+      const double tr = (vp[0] + vp[4]) + vp[8];
+      const double sh = tr / S[MM_S_NDOF];
+      A[0] += sh;
+      A[4] += sh;
+      A[8] += sh;
+    }
+    expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], S + MM_S_VMAT);
+  } else {
+    const double f = exp(((-((1.0 + 3.0 / S[MM_S_NDOF]) * vp[0])) * S[MM_S_TIMESTEP]) / 2.0);
+    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] = (i % 4 == 0) ? f : 0.0;
+  }
   S[MM_S_VMAT_ON] = 1.0;
+}
+
+// LangevinBarostat.pre/post bookkeeping (npt.py:374-392): energies at the start of the call.
+__device__ void baro_call_begin(double* S) {
+  if (baro_langevin(S)) {
+    S[MM_S_BARO_EPOT0] = S[MM_S_EPOT];
+    S[MM_S_BARO_EKIN0] = S[MM_S_EKIN];
+  }
+}
+
+// ... and the econs correction at its end: += epot0 - epot1 + ekin0 - ekin1.
+__device__ void baro_call_end(double* S) {
+#pragma clang fp contract(off)
+  if (baro_langevin(S))
+    S[MM_S_ECONS_BARO] += ((S[MM_S_BARO_EPOT0] - S[MM_S_EPOT]) + S[MM_S_BARO_EKIN0]) - S[MM_S_EKIN];
+}
+
+// LangevinThermostat.pre/post bookkeeping (nvt.py:197-209) for the two updates of a step made
+// in the node passes: econs += ekin0 - ekin1 for each.
+__device__ void langevin_thermo_econs(double* S, double ekin_before_pre, const double (&nd)[kNodeSums]) {
+#pragma clang fp contract(off)
+  S[MM_S_ECONS_THERMO] += ekin_before_pre - nd[14];
+  S[MM_S_ECONS_THERMO] += nd[13] - S[MM_S_EKIN];
 }
 
 __device__ void clear_vtransforms(double* S) {
@@ -455,9 +607,11 @@ __device__ double econs_correction(const double* S) {
 #pragma clang fp contract(off)
   const bool th = S[MM_S_THERMO] != 0.0, ba = S[MM_S_BARO] != 0.0;
   if (th && ba) {
+    const double corr = S[MM_S_ECONS_THERMO] + S[MM_S_ECONS_BARO];
+    if (!thermo_nhc(S)) return corr;
+    // particle NHC thermostat also thermostats the barostat (npt.py:139-148)
     const double kt = S[MM_S_BOLTZMANN] * S[MM_S_THERMO_TEMP];
-    return (S[MM_S_ECONS_THERMO] + S[MM_S_ECONS_BARO]) +
-           S[MM_S_BARO_NDOF] * kt * S[MM_S_CHAIN_POS];
+    return corr + S[MM_S_BARO_NDOF] * kt * S[MM_S_CHAIN_POS];
   }
   if (th) return S[MM_S_ECONS_THERMO];
   if (ba) return S[MM_S_ECONS_BARO];
@@ -516,7 +670,7 @@ __device__ void take_force(double* S, const double (&f)[10], bool with_vtens) {
   if (f[0] != f[0]) S[MM_S_NAN] = 1.0;
 }
 
-__device__ void take_nodes(double* S, const double (&nd)[13]) {
+__device__ void take_nodes(double* S, const double (&nd)[kNodeSums]) {
   S[MM_S_EKIN] = nd[0];
   for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] = nd[1 + q];
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
@@ -567,10 +721,13 @@ __device__ __forceinline__ bool phase_uses_nodes(int phase) {
          phase == kPhRmsd;
 }
 
-// The scalar update of one phase on the scalar block S (thread 0 of the system's last block).
-__device__ void run_phase(int phase, double* S, const double (&f)[10], const double (&nd)[13],
-                          int with_vtens) {
-  const bool thermo = S[MM_S_THERMO] != 0.0;
+// The scalar update of one phase on the scalar block S of system sys.
+__device__ void run_phase(int phase, double* S, const double (&f)[10],
+                          const double (&nd)[kNodeSums], int with_vtens, int sys) {
+  const bool nhc = thermo_nhc(S);
+  const bool lt = S[MM_S_THERMO] == (double)MM_THERMO_LANGEVIN;
+  // the barostat's pressure contribution enters the chain only for MTK (npt.py:109-113)
+  const bool g1_on = S[MM_S_BARO] == (double)MM_BARO_MTK;
   switch (phase) {
     case kPhInitForce:
       take_force(S, f, with_vtens != 0);
@@ -595,6 +752,13 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10], const dou
     case kPhNvtEnd: {
       clear_vtransforms(S);
       take_force(S, f, true);
+      if (lt) {
+        const double ekin0 = S[MM_S_EKIN];
+        take_nodes(S, nd);
+        langevin_thermo_econs(S, ekin0, nd);
+        properties(S, true);
+        break;
+      }
       take_nodes(S, nd);
       double ekin = S[MM_S_EKIN];
       const double fac = nhc_chain(S, &ekin, false, 0.0);
@@ -607,14 +771,15 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10], const dou
     }
     case kPhBaroPre:
       // TBC.pre: chainvel0 = chain.vel[0] before the barostat (npt.py:101-107)
-      S[MM_S_HAS_CHAINVEL0] = thermo ? 1.0 : 0.0;
-      S[MM_S_CHAINVEL0] = thermo ? S[MM_S_CHAIN_VEL] : 0.0;
+      S[MM_S_HAS_CHAINVEL0] = nhc ? 1.0 : 0.0;
+      S[MM_S_CHAINVEL0] = nhc ? S[MM_S_CHAIN_VEL] : 0.0;
       // a pending thermostat factor scales m v (x) v by its square
       if (S[MM_S_VSCALE_A_ON] != 0.0) {
         const double fac = S[MM_S_VSCALE_A];
         for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
       }
-      baro_part1(S);
+      baro_call_begin(S);
+      baro_part1(S, sys, 0);
       break;
     case kPhBaroMid:
       take_force(S, f, true);
@@ -623,11 +788,13 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10], const dou
     case kPhBaroEndPre: {
       clear_vtransforms(S);
       take_nodes(S, nd);
-      update_baro_vel(S);
-      if (thermo) {
-        const double g1 = add_press_cont(S);
+      update_baro_vel(S, sys, 0, 1);
+      baro_call_end(S);
+      // a Langevin thermostat's pre update follows in the kick-drift pass
+      if (nhc) {
+        const double g1 = g1_on ? add_press_cont(S) : 0.0;
         double ekin = S[MM_S_EKIN];
-        const double fac = nhc_chain(S, &ekin, true, g1);
+        const double fac = nhc_chain(S, &ekin, g1_on, g1);
         S[MM_S_EKIN] = ekin;
         set_vscale(S, MM_S_VSCALE_B, fac);
       }
@@ -636,27 +803,34 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10], const dou
     case kPhMainEndNpt: {
       clear_vtransforms(S);
       take_force(S, f, true);
+      const double ekin0 = S[MM_S_EKIN];
       take_nodes(S, nd);
-      if (thermo) {
-        const double g1 = add_press_cont(S);
+      if (nhc) {
+        const double g1 = g1_on ? add_press_cont(S) : 0.0;
         double ekin = S[MM_S_EKIN];
-        const double fac = nhc_chain(S, &ekin, true, g1);
+        const double fac = nhc_chain(S, &ekin, g1_on, g1);
         S[MM_S_EKIN] = ekin;
         set_vscale(S, MM_S_VSCALE_A, fac);
         for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
         S[MM_S_ECONS_THERMO] = nhc_econs(S);
+      } else if (lt) {
+        langevin_thermo_econs(S, ekin0, nd);
       }
       // TBC.post: chainvel0 after the thermostat (npt.py:128-133)
-      S[MM_S_HAS_CHAINVEL0] = thermo ? 1.0 : 0.0;
-      S[MM_S_CHAINVEL0] = thermo ? S[MM_S_CHAIN_VEL] : 0.0;
-      baro_part1(S);
+      S[MM_S_HAS_CHAINVEL0] = nhc ? 1.0 : 0.0;
+      S[MM_S_CHAINVEL0] = nhc ? S[MM_S_CHAIN_VEL] : 0.0;
+      baro_call_begin(S);
+      baro_part1(S, sys, 1);
       break;
     }
     case kPhBaroEndPost:
       clear_vtransforms(S);
       take_nodes(S, nd);
-      update_baro_vel(S);
-      S[MM_S_ECONS_BARO] = baro_ekin(S) + S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
+      update_baro_vel(S, sys, 1, 1);
+      if (baro_langevin(S))
+        baro_call_end(S);
+      else
+        S[MM_S_ECONS_BARO] = baro_ekin(S) + S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
       properties(S, true);
       break;
     case kPhRmsd: {
@@ -685,11 +859,11 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
   __shared__ int is_last;
   const int sys = blockIdx.y;
   const int G = gridDim.x, gi = blockIdx.x;
-  double f[10], nd[13];
+  double f[10], nd[kNodeSums];
 #pragma unroll
   for (int q = 0; q < 10; ++q) f[q] = 0.0;
 #pragma unroll
-  for (int q = 0; q < 13; ++q) nd[q] = 0.0;
+  for (int q = 0; q < kNodeSums; ++q) nd[q] = 0.0;
   if (phase_uses_force(phase)) {
     const int b0 = fsys_ptr[sys], cnt = fsys_ptr[sys + 1] - b0;
     const int lo = (int)((int64_t)cnt * gi / G), hi = (int)((int64_t)cnt * (gi + 1) / G);
@@ -698,13 +872,13 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
   if (phase_uses_nodes(phase)) {
     const int b0 = nsys_ptr[sys], cnt = nsys_ptr[sys + 1] - b0;
     const int lo = (int)((int64_t)cnt * gi / G), hi = (int)((int64_t)cnt * (gi + 1) / G);
-    reduce_blocks<13>(npart, nbn, nullptr, b0 + lo, hi - lo, nd);
+    reduce_blocks<kNodeSums>(npart, nbn, nullptr, b0 + lo, hi - lo, nd);
   }
   if (G > 1) {
     if (threadIdx.x == 0) {
       double* sl = slices + ((int64_t)sys * G + gi) * kSliceQ;
       for (int q = 0; q < 10; ++q) sl[q] = f[q];
-      for (int q = 0; q < 13; ++q) sl[10 + q] = nd[q];
+      for (int q = 0; q < kNodeSums; ++q) sl[10 + q] = nd[q];
       __threadfence();
       const unsigned int prev = atomicAdd(&counters[sys], 1u);
       is_last = prev == (unsigned int)(G - 1);
@@ -723,12 +897,12 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
 #pragma unroll
     for (int q = 0; q < 10; ++q) f[q] = tot[q];
 #pragma unroll
-    for (int q = 0; q < 13; ++q) nd[q] = tot[10 + q];
+    for (int q = 0; q < kNodeSums; ++q) nd[q] = tot[10 + q];
   }
   double* Sg = S0 + (int64_t)sys * MM_S_COUNT;
   for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sl[q] = Sg[q];
   __syncthreads();
-  if (threadIdx.x == 0) run_phase(phase, Sl, f, nd, with_vtens);
+  if (threadIdx.x == 0) run_phase(phase, Sl, f, nd, with_vtens, sys);
   __syncthreads();
   for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sg[q] = Sl[q];
 }
@@ -816,6 +990,7 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
     case kVV: MM_NODE(kVV); break;
     case kApplyV: MM_NODE(kApplyV); break;
     case kRmsd: MM_NODE(kRmsd); break;
+    case kKickDriftLT: MM_NODE(kKickDriftLT); break;
   }
 #undef MM_NODE
   return hipGetLastError();
@@ -837,21 +1012,24 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
   return hipGetLastError();
 }
 
-// force evaluation: cells, then gather (+ second half kick when kick)
-hipError_t force(mm_md* md, bool kick, hipStream_t s) {
+// force evaluation: cells, then gather (+ second half kick when kick, + the Langevin
+// thermostat's post update when lt)
+hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s) {
   ForceArgs a = force_args(md);
   hipError_t e = launch_cells(a, s);
   if (e != hipSuccess) return e;
   if (md->N > 0) {
-    if (kick) {
-      hipLaunchKernelGGL((gather_kick_kernel<true>), dim3(md->nbn), dim3(kNodeBlock), 0, s,
-                         a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,
-                         md->node_blk_sys, md->npart);
-    } else {
-      hipLaunchKernelGGL((gather_kick_kernel<false>), dim3(md->nbn), dim3(kNodeBlock), 0, s,
-                         a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,
-                         md->node_blk_sys, md->npart);
-    }
+#define MM_GATHER(K, T)                                                                      \
+  hipLaunchKernelGGL((gather_kick_kernel<K, T>), dim3(md->nbn), dim3(kNodeBlock), 0, s,       \
+                     a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,      \
+                     md->node_blk_sys, md->npart)
+    if (kick && lt)
+      MM_GATHER(true, true);
+    else if (kick)
+      MM_GATHER(true, false);
+    else
+      MM_GATHER(false, false);
+#undef MM_GATHER
     e = hipGetLastError();
   }
   return e;
@@ -866,7 +1044,7 @@ hipError_t force(mm_md* md, bool kick, hipStream_t s) {
 // The barostat call MTKBarostat.baro after its first scalar phase.
 hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s) {
   MM_TRY(node(md, kScalePos, s));
-  MM_TRY(force(md, false, s));
+  MM_TRY(force(md, false, false, s));
   MM_TRY(scalar(md, kPhBaroMid, 1, s));
   MM_TRY(node(md, kScaleVel, s));
   MM_TRY(scalar(md, end_phase, 1, s));
@@ -874,21 +1052,27 @@ hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s) {
 }
 
 hipError_t enqueue_step(mm_md* md, hipStream_t s) {
+  const bool lt = md->thermo == MM_THERMO_LANGEVIN;
+  const int kick_drift = lt ? kKickDriftLT : kKickDrift;
   if (md->baro) {
     MM_TRY(scalar(md, kPhBaroPre, 1, s));
     MM_TRY(baro_call(md, kPhBaroEndPre, s));
-    MM_TRY(node(md, kKickDrift, s));
-    MM_TRY(force(md, true, s));
+    MM_TRY(node(md, kick_drift, s));
+    MM_TRY(force(md, true, lt, s));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
     MM_TRY(baro_call(md, kPhBaroEndPost, s));
-  } else if (md->thermo) {
+  } else if (md->thermo == MM_THERMO_NHC) {
     MM_TRY(scalar(md, kPhNvtPre, 1, s));
     MM_TRY(node(md, kKickDrift, s));
-    MM_TRY(force(md, true, s));
+    MM_TRY(force(md, true, false, s));
+    MM_TRY(scalar(md, kPhNvtEnd, 1, s));
+  } else if (lt) {
+    MM_TRY(node(md, kKickDriftLT, s));
+    MM_TRY(force(md, true, true, s));
     MM_TRY(scalar(md, kPhNvtEnd, 1, s));
   } else {
     MM_TRY(node(md, kKickDrift, s));
-    MM_TRY(force(md, true, s));
+    MM_TRY(force(md, true, false, s));
     MM_TRY(scalar(md, kPhNveEnd, 1, s));
   }
   return hipSuccess;
@@ -918,9 +1102,9 @@ int mm_md_create_batch(mm_md** out, mm_field* field, int32_t nsys, const int64_t
     return set_error(MM_ERR_ARG, "field holds %d systems, not %d", field->nsys, nsys);
   if (field->N > 0 && (!pos || !vel || !gpos || !masses || !posold))
     return set_error(MM_ERR_ARG, "missing node buffer");
-  if (thermo != MM_THERMO_NONE && thermo != MM_THERMO_NHC)
+  if (thermo != MM_THERMO_NONE && thermo != MM_THERMO_NHC && thermo != MM_THERMO_LANGEVIN)
     return set_error(MM_ERR_ARG, "unknown thermostat %d", thermo);
-  if (baro != MM_BARO_NONE && baro != MM_BARO_MTK)
+  if (baro != MM_BARO_NONE && baro != MM_BARO_MTK && baro != MM_BARO_LANGEVIN)
     return set_error(MM_ERR_ARG, "unknown barostat %d", baro);
   if (sys_node_start[0] != 0 || sys_node_start[nsys] != field->N)
     return set_error(MM_ERR_ARG, "node ranges must cover [0, %lld)", (long long)field->N);
@@ -961,6 +1145,7 @@ int mm_md_create_batch(mm_md** out, mm_field* field, int32_t nsys, const int64_t
   }
   md->slices = std::min(kScalarSlices, std::max(1, (most + 255) / 256));
   if (e == hipSuccess) e = hipMalloc((void**)&md->npart, sizeof(double) * kNodePartial * md->nbn);
+  if (e == hipSuccess) e = hipMemset(md->npart, 0, sizeof(double) * kNodePartial * md->nbn);
   if (e == hipSuccess)
     e = hipMalloc((void**)&md->slice_buf, sizeof(double) * kSliceQ * md->slices * nsys);
   if (e == hipSuccess) e = hipMalloc((void**)&md->counters, sizeof(unsigned int) * nsys);
@@ -998,7 +1183,7 @@ int mm_md_init_force(mm_md* md, int with_vtens, void* stream) {
   Dev dev(md->field->device);
   int rc = begin(md, stream);
   if (rc) return rc;
-  hipError_t e = force(md, false, md->stream);
+  hipError_t e = force(md, false, false, md->stream);
   if (e == hipSuccess) e = scalar(md, kPhInitForce, with_vtens, md->stream);
   if (e != hipSuccess) return hip_error(e, "mm_md_init_force");
   return end(md, stream);

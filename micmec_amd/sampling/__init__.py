@@ -4,14 +4,16 @@ from .iterative import (Iterative, StateItem, AttributeStateItem, PosStateItem,
                         TemperatureStateItem, VolumeStateItem, DomainStateItem,
                         ConsErrStateItem, EPotContribStateItem, Hook)
 from .verlet import VerletIntegrator, VerletHook, VerletScreenLog, ConsErrTracker
-from .nvt import NHChain, NHCThermostat, NHCAttributeStateItem
-from .npt import TBCombination, MTKBarostat, MTKAttributeStateItem
+from .nvt import NHChain, NHCThermostat, LangevinThermostat, NHCAttributeStateItem
+from .npt import TBCombination, MTKBarostat, LangevinBarostat, MTKAttributeStateItem
 from .dof import DOF, CartesianDOF, BaseCellDOF, FullCellDOF, StrainCellDOF
 from .opt import OptScreenLog, BaseOptimizer, CGOptimizer, ConjugateGradientSearch
 
 __all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
            "TemperatureStateItem", "VolumeStateItem", "DomainStateItem", "ConsErrStateItem",
            "EPotContribStateItem", "Hook", "VerletIntegrator", "VerletHook", "VerletScreenLog",
-           "ConsErrTracker", "NHChain", "NHCThermostat", "NHCAttributeStateItem",
-           "TBCombination", "MTKBarostat", "MTKAttributeStateItem", "DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF", "StrainCellDOF",
+           "ConsErrTracker", "NHChain", "NHCThermostat", "LangevinThermostat",
+           "NHCAttributeStateItem", "TBCombination", "MTKBarostat", "LangevinBarostat",
+           "MTKAttributeStateItem", "DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF",
+           "StrainCellDOF",
            "OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch"]

@@ -1,22 +1,23 @@
-"""MTK barostat and the thermostat-barostat combination (micmec/sampling/npt.py:52-177, 513-787).
+"""Barostats and the thermostat-barostat combination (micmec/sampling/npt.py:52-177, 295-787).
 
 ``MTKBarostat`` (anisotropic, full 3x3 cell fluctuation, no volume constraint, no barostat
-thermostat) and ``TBCombination`` keep the reference constructors. Host-side ``init`` mirrors the
-reference (domain symmetrisation at construction, momentum cleaning, barostat mass, random
-symmetric barostat velocity); the per-step barostat updates, including the extra force
-evaluation of each ``baro`` call, run inside the device integrator.
+thermostat), ``LangevinBarostat`` (anisotropic or isotropic, with or without volume constraint)
+and ``TBCombination`` keep the reference constructors. Host-side ``init`` mirrors the reference
+(domain symmetrisation at construction, momentum cleaning, barostat mass, random symmetric
+barostat velocity); the per-step barostat updates, including the extra force evaluation of each
+``baro`` call and the Langevin barostat's random force, run inside the device integrator.
 """
 
 import numpy as np
 
 from ..units import boltzmann, femtosecond
 from .iterative import StateItem
-from .nvt import NHCThermostat
+from .nvt import LangevinThermostat, NHCThermostat
 from .utils import (clean_momenta, domain_symmetrize, get_ndof_baro, get_ndof_internal_md,
                     get_random_vel_press)
 from .verlet import VerletHook
 
-__all__ = ["TBCombination", "MTKBarostat", "MTKAttributeStateItem"]
+__all__ = ["TBCombination", "MTKBarostat", "LangevinBarostat", "MTKAttributeStateItem"]
 
 
 class TBCombination(VerletHook):
@@ -38,14 +39,21 @@ class TBCombination(VerletHook):
         VerletHook.__init__(self, start, min(self.step_thermo, self.step_baro))
 
     def init(self, iterative):
+        set_ndof = iterative.ndof is not None
         self.thermostat.init(iterative)
         self.barostat.init(iterative)
+        # ndof = 3N when a stochastic hook lets the centre-of-mass momentum fluctuate and the
+        # caller did not fix ndof (npt.py:87-95)
+        p_cm_fluct = isinstance(self.thermostat, LangevinThermostat) or \
+            isinstance(self.barostat, LangevinBarostat)
+        if not set_ndof and p_cm_fluct:
+            iterative.ndof = iterative.pos.size
         self.chainvel0 = None
         self.G1_add = None
 
     def verify(self):
-        return isinstance(self.thermostat, NHCThermostat) and \
-            isinstance(self.barostat, MTKBarostat)
+        return isinstance(self.thermostat, (NHCThermostat, LangevinThermostat)) and \
+            isinstance(self.barostat, (MTKBarostat, LangevinBarostat))
 
     def pre(self, iterative):
         raise NotImplementedError("TBC updates run inside the device integrator.")
@@ -110,6 +118,61 @@ class MTKBarostat(VerletHook):
 
     def _compute_ekin_baro(self):
         return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
+
+
+class LangevinBarostat(VerletHook):
+    """Langevin barostat (npt.py:295-510; Feller, Zhang, Pastor and Brooks, J. Chem. Phys. 103,
+    4613, 1995): the MTK cell update with a friction exp(-dt / (8 timecon)) on both sides of
+    every barostat-velocity update and a random symmetric force R of variance
+    2 W k_B T / (dt timecon), drawn on the device (Philox4x64-10 keyed by ``seed``)."""
+    name = "Langevin"
+    kind = "stochastic"
+    method = "barostat"
+    _device_hook = True
+
+    def __init__(self, mmf, temp, press, start=0, step=1, timecon=1000 * femtosecond,
+                 anisotropic=True, vol_constraint=False, seed=None):
+        self.temp = temp
+        self.press = press
+        self.timecon = timecon
+        self.anisotropic = anisotropic
+        self.vol_constraint = vol_constraint
+        self.seed = seed
+        self.dim = mmf.system.domain.nvec
+        if self.dim != 3:
+            raise ValueError("The Langevin barostat needs a 3D periodic domain.")
+        self.baro_ndof = get_ndof_baro(self.dim, self.anisotropic, self.vol_constraint)
+        if self.anisotropic:
+            domain_symmetrize(mmf)
+        self.domain = np.asarray(mmf.system.domain.rvecs).copy()
+        self.econs_correction = 0.0
+        VerletHook.__init__(self, start, step)
+
+    def init(self, iterative):
+        self.timestep_press = iterative.timestep
+        clean_momenta(iterative.pos, iterative.vel, iterative.masses,
+                      iterative.mmf.system.domain)
+        if iterative.ndof is None:
+            iterative.ndof = iterative.pos.size
+        self.mass_press = (iterative.ndof + 3) / 3 * boltzmann * self.temp * (
+            self.timecon / (2 * np.pi)) ** 2
+        self.vel_press = get_random_vel_press(self.mass_press, self.temp)
+        if self.vol_constraint:
+            self.vel_press -= np.trace(self.vel_press) / 3 * np.eye(3)
+        if not self.anisotropic:
+            self.vel_press = self.vel_press[0][0]
+        # gpos and vtens after the symmetrisation (npt.py:368-372)
+        iterative._init_force(with_vtens=True)
+
+    def random_force_scale(self):
+        """sqrt(2 W k_B T / (dt timecon)) in the reference's operation order (npt.py:492-497)."""
+        return np.sqrt(2 * self.mass_press * boltzmann * self.temp /
+                       (self.timestep_press * self.timecon))
+
+    def pre(self, iterative, chainvel0=None):
+        raise NotImplementedError("Langevin barostat updates run inside the device integrator.")
+
+    post = pre
 
 
 class MTKAttributeStateItem(StateItem):

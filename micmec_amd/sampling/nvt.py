@@ -1,10 +1,17 @@
-"""Nose-Hoover chain thermostat (micmec/sampling/nvt.py:361-557).
+"""Thermostats: Nose-Hoover chain (micmec/sampling/nvt.py:361-557) and Langevin
+(micmec/sampling/nvt.py:165-218).
 
-``NHCThermostat`` keeps the reference constructor and attributes. Its ``init`` runs on the host
-with numpy (momentum cleaning, chain masses, random chain velocities in the reference's draw
-order); its per-step ``pre``/``post`` updates run inside the device integrator
-(``micmec_amd/csrc/mm_md.hip``, ``nhc_chain``), whose chain state is mirrored back into
-``chain.pos`` / ``chain.vel`` whenever host hooks run.
+``NHCThermostat`` and ``LangevinThermostat`` keep the reference constructors and attributes.
+Their ``init`` runs on the host with numpy (momentum cleaning, chain masses, random chain
+velocities in the reference's draw order); their per-step ``pre``/``post`` updates run inside
+the device integrator (``micmec_amd/csrc/mm_md.hip``), whose state is mirrored back into the
+hook (``chain.pos`` / ``chain.vel``, ``econs_correction``) whenever host hooks run.
+
+The Langevin noise is drawn on the device from a counter-based generator (Philox4x64-10 keyed
+by the hook's ``seed``) instead of numpy's global stream: every node draws its own normals for
+every hook call, in the node passes that already update the velocities. A seeded run is
+reproducible; its noise differs from the reference's draws, so agreement with the reference
+is statistical (the same update applied to the same distribution of noise).
 """
 
 import numpy as np
@@ -14,7 +21,7 @@ from .iterative import StateItem
 from .utils import clean_momenta, get_ndof_internal_md
 from .verlet import VerletHook
 
-__all__ = ["NHChain", "NHCThermostat", "NHCAttributeStateItem"]
+__all__ = ["NHChain", "NHCThermostat", "LangevinThermostat", "NHCAttributeStateItem"]
 
 
 class NHChain(object):
@@ -106,6 +113,44 @@ class NHCThermostat(VerletHook):
 
     def pre(self, iterative, G1_add=None):
         raise NotImplementedError("NHC updates run inside the device integrator.")
+
+    post = pre
+
+
+def draw_seed():
+    """A 62-bit key for the device noise, from numpy's global stream (drawn after every hook's
+    init, so the reference's draws at initialisation keep their order)."""
+    return int(np.random.randint(0, 2 ** 62, dtype=np.int64))
+
+
+class LangevinThermostat(VerletHook):
+    """Langevin thermostat (nvt.py:165-218): at every pre and post call
+    v <- c1 v + sqrt((1 - c1^2) T k_B / m) xi with c1 = exp(-dt / (2 timecon)) and xi standard
+    normals, and econs_correction += ekin before - ekin after (Bussi and Parrinello,
+    Phys. Rev. E 75, 056707, 2007)."""
+    name = "Langevin"
+    kind = "stochastic"
+    method = "thermostat"
+    _device_hook = True
+
+    def __init__(self, temp, start=0, timecon=100 * femtosecond, seed=None):
+        self.temp = temp
+        self.timecon = timecon
+        self.seed = seed
+        VerletHook.__init__(self, start, 1)
+
+    def init(self, iterative):
+        # the external momenta must be zero (nvt.py:193-196)
+        clean_momenta(iterative.pos, iterative.vel, iterative.masses,
+                      iterative.mmf.system.domain)
+
+    def device_scalars(self, timestep):
+        """c1 and (1 - c1^2) T k_B in the reference's operation order (nvt.py:212-215)."""
+        c1 = np.exp(-timestep / self.timecon / 2)
+        return c1, (1.0 - c1 ** 2) * self.temp * boltzmann
+
+    def pre(self, iterative, G1_add=None):
+        raise NotImplementedError("Langevin updates run inside the device integrator.")
 
     post = pre
 

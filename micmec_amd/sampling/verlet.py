@@ -4,14 +4,15 @@
 attributes. Initialisation runs on the host exactly as in the reference (random velocities from
 numpy's global stream, momentum cleaning, the hooks' ``init``), the force evaluations included
 run on the GPU, and then every step runs in the device engine of ``libmicmec_amd.so``
-(``mm_md_step``): kick, drift, force, kick and the NHC/MTK hook updates, with the scalar state of
+(``mm_md_step``): kick, drift, force, kick and the thermostat/barostat updates, with the scalar state of
 the integrator and its hooks kept in device memory. Steps between two host-hook calls are issued
 as one batch of HIP-graph replays; host attributes (``pos``, ``vel``, ``epot``, ``econs``, ...)
 are refreshed before every hook call and after every ``run``/``propagate``.
 
 Supported force fields: one ``ForcePartMechanical`` plus at most one ``ForcePartPressure``.
-Supported device hooks: ``NHCThermostat``, ``MTKBarostat`` and their ``TBCombination``. Host
-hooks (screen log, writers, user hooks) run at their own cadence.
+Supported device hooks: ``NHCThermostat``, ``LangevinThermostat``, ``MTKBarostat``,
+``LangevinBarostat`` and their ``TBCombination``. Host hooks (screen log, writers, user hooks)
+run at their own cadence.
 """
 
 import ctypes
@@ -252,15 +253,15 @@ class VerletIntegrator(Iterative):
         self.hooks = hooks
 
     def _device_kinds(self):
-        from .npt import MTKBarostat, TBCombination
-        from .nvt import NHCThermostat
+        from .npt import LangevinBarostat, MTKBarostat, TBCombination
+        from .nvt import LangevinThermostat, NHCThermostat
         thermo = baro = None
         for hook in self.hooks:
             if isinstance(hook, TBCombination):
                 thermo, baro = hook.thermostat, hook.barostat
-            elif isinstance(hook, NHCThermostat):
+            elif isinstance(hook, (NHCThermostat, LangevinThermostat)):
                 thermo = hook
-            elif isinstance(hook, MTKBarostat):
+            elif isinstance(hook, (MTKBarostat, LangevinBarostat)):
                 baro = hook
             elif isinstance(hook, VerletHook):
                 raise NotImplementedError("%s has no device implementation" %
@@ -353,10 +354,15 @@ class VerletIntegrator(Iterative):
 
     # ------------------------------------------------------------------ initialisation
     def initialize(self):
+        from .npt import LangevinBarostat
+        from .nvt import LangevinThermostat, draw_seed
         thermo, baro = self._device_kinds()
         self._thermo, self._baro = thermo, baro
-        self._engine = _DeviceEngine(self, L.THERMO_NHC if thermo else L.THERMO_NONE,
-                                     L.BARO_MTK if baro else L.BARO_NONE)
+        lt = isinstance(thermo, LangevinThermostat)
+        lb = isinstance(baro, LangevinBarostat)
+        self._engine = _DeviceEngine(
+            self, L.THERMO_NONE if thermo is None else (L.THERMO_LANGEVIN if lt else L.THERMO_NHC),
+            L.BARO_NONE if baro is None else (L.BARO_LANGEVIN if lb else L.BARO_MTK))
         self._uploaded = False
         self._S_version = 0
         S = np.zeros(L.S_COUNT)
@@ -381,21 +387,47 @@ class VerletIntegrator(Iterative):
         S = self._S.copy()
         S[L.S_NDOF] = self.ndof
         S[L.S_COUNTER] = self.counter
-        if thermo is not None:
+        # keys of the device noise, drawn after every hook's init
+        for hook in (thermo if lt else None, baro if lb else None):
+            if hook is not None and hook.seed is None:
+                hook.seed = draw_seed()
+
+        def put_seed(slot, seed):
+            seed = int(seed) & ((1 << 64) - 1)
+            S[slot] = float(seed & 0xFFFFFFFF)
+            S[slot + 1] = float(seed >> 32)
+
+        if lt:
+            c1, amp = thermo.device_scalars(self.timestep)
+            S[L.S_THERMO] = L.THERMO_LANGEVIN
+            S[L.S_THERMO_TEMP] = thermo.temp
+            S[L.S_LT_C1] = c1
+            S[L.S_LT_A] = amp
+            put_seed(L.S_LT_SEED, thermo.seed)
+        elif thermo is not None:
             ch = thermo.chain
-            S[L.S_THERMO] = 1.0
+            S[L.S_THERMO] = L.THERMO_NHC
             S[L.S_CHAIN_LEN] = ch.length
+            S[L.S_CHAIN_NDOF] = ch.ndof
             S[L.S_THERMO_TEMP] = thermo.temp
             S[L.S_CHAIN_MASS:L.S_CHAIN_MASS + ch.length] = ch.masses
             S[L.S_CHAIN_POS:L.S_CHAIN_POS + ch.length] = ch.pos
             S[L.S_CHAIN_VEL:L.S_CHAIN_VEL + ch.length] = ch.vel
         if baro is not None:
-            S[L.S_BARO] = 1.0
+            S[L.S_BARO] = L.BARO_LANGEVIN if lb else L.BARO_MTK
             S[L.S_BARO_TEMP] = baro.temp
             S[L.S_BARO_PRESS] = baro.press
             S[L.S_MASS_PRESS] = baro.mass_press
             S[L.S_BARO_NDOF] = baro.baro_ndof
-            S[L.S_VEL_PRESS:L.S_VEL_PRESS + 9] = np.asarray(baro.vel_press).ravel()
+            S[L.S_BARO_ANISO] = 1.0 if baro.anisotropic else 0.0
+            S[L.S_BARO_VOLCON] = 1.0 if baro.vol_constraint else 0.0
+            vp = np.zeros(9)
+            vp[:np.size(baro.vel_press)] = np.asarray(baro.vel_press).ravel()
+            S[L.S_VEL_PRESS:L.S_VEL_PRESS + 9] = vp
+            if lb:
+                S[L.S_LB_TIMECON] = baro.timecon
+                S[L.S_LB_RSCALE] = baro.random_force_scale()
+                put_seed(L.S_LB_SEED, baro.seed)
         self._engine.put_scalars(S)
         self._engine.pos.copy_(torch.from_numpy(np.ascontiguousarray(self._pos)))
         self._engine.vel.copy_(torch.from_numpy(np.ascontiguousarray(self._vel)))
@@ -470,12 +502,16 @@ class VerletIntegrator(Iterative):
     def _mirror_hooks(self):
         S = self._S
         if self._thermo is not None:
-            ch = self._thermo.chain
-            ch.pos = S[L.S_CHAIN_POS:L.S_CHAIN_POS + ch.length].copy()
-            ch.vel = S[L.S_CHAIN_VEL:L.S_CHAIN_VEL + ch.length].copy()
+            ch = getattr(self._thermo, "chain", None)
+            if ch is not None:
+                ch.pos = S[L.S_CHAIN_POS:L.S_CHAIN_POS + ch.length].copy()
+                ch.vel = S[L.S_CHAIN_VEL:L.S_CHAIN_VEL + ch.length].copy()
             self._thermo.econs_correction = float(S[L.S_ECONS_THERMO])
         if self._baro is not None:
-            self._baro.vel_press = S[L.S_VEL_PRESS:L.S_VEL_PRESS + 9].reshape(3, 3).copy()
+            if self._baro.anisotropic:
+                self._baro.vel_press = S[L.S_VEL_PRESS:L.S_VEL_PRESS + 9].reshape(3, 3).copy()
+            else:
+                self._baro.vel_press = float(S[L.S_VEL_PRESS])
             self._baro.econs_correction = float(S[L.S_ECONS_BARO])
         for hook in self.hooks:
             if getattr(hook, "name", None) == "TBCombination":

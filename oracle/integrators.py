@@ -1,4 +1,5 @@
-"""CPU restatement of MicMec's velocity Verlet with NHC and MTK hooks (test infrastructure only).
+"""CPU restatement of MicMec's velocity Verlet with its thermostat and barostat hooks (test
+infrastructure only).
 
 Follows, in the same order of operations and random draws:
   micmec/sampling/verlet.py:71-190        VerletIntegrator (init, propagate, properties)
@@ -6,6 +7,9 @@ Follows, in the same order of operations and random draws:
   micmec/sampling/nvt.py:361-532          NHChain and NHCThermostat
   micmec/sampling/npt.py:52-177           TBCombination
   micmec/sampling/npt.py:513-757          MTKBarostat (anisotropic, no volume constraint)
+  micmec/sampling/nvt.py:165-218          LangevinThermostat
+  micmec/sampling/npt.py:295-510          LangevinBarostat (anisotropic or isotropic, with or
+                                          without volume constraint)
   micmec/sampling/utils.py:28-174, 322-383, 450-501  init helpers
 The force field is any callable ``force(pos, rvecs) -> (epot, gpos, vtens)``.
 """
@@ -13,6 +17,7 @@ The force field is any callable ``force(pos, rvecs) -> (epot, gpos, vtens)``.
 import numpy as np
 
 from .constants import boltzmann, femtosecond
+from .philox import barostat_noise, thermostat_noise
 
 
 # ---------------------------------------------------------------- sampling/utils.py
@@ -151,11 +156,15 @@ class NHChain(object):
 
 
 class OracleMD(object):
-    """Velocity Verlet with optional NHC thermostat and MTK barostat (combined as in TBC).
+    """Velocity Verlet with an optional thermostat (NHC or Langevin) and barostat (MTK or
+    Langevin), combined as in TBC.
 
     Parameters mirror the reference constructors: ``thermo = dict(temp, timecon, chainlength)``
-    and ``baro = dict(temp, press, timecon)``. ``vel0``/``chain_vel0``/``vel_press0`` skip the
-    corresponding random draws exactly as the reference does.
+    for NHC or ``dict(kind="langevin", temp, timecon, seed)``; ``baro = dict(temp, press,
+    timecon)`` for MTK or ``dict(kind="langevin", temp, press, timecon, anisotropic,
+    vol_constraint, seed)``. ``vel0``/``chain_vel0``/``vel_press0`` skip the corresponding random
+    draws exactly as the reference does. The Langevin noise is the counter-based noise of
+    ``oracle.philox`` for the given seeds (the reference draws it from numpy's global stream).
     """
 
     def __init__(self, force, pos, rvecs, masses, timestep, temp0=300.0, vel0=None,
@@ -168,13 +177,19 @@ class OracleMD(object):
         self.ext_press = ext_press
         pos = pos.copy()
         rvecs = rvecs.copy()
-        self.baro = None
-        if baro is not None:
-            # MTKBarostat.__init__ symmetrises the domain before the integrator copies pos
-            # (npt.py:566-568 via utils.py:346-383)
-            pos, rvecs = domain_symmetrize(pos, rvecs)
-            self.baro = dict(baro)
-            self.baro_ndof = 6
+        self.thermo = None if thermo is None else dict(thermo)
+        self.baro = None if baro is None else dict(baro)
+        self.thermo_kind = None if thermo is None else self.thermo.get("kind", "nhc")
+        self.baro_kind = None if baro is None else self.baro.get("kind", "mtk")
+        if self.baro is not None:
+            self.aniso = self.baro.get("anisotropic", True)
+            self.volcon = self.baro.get("vol_constraint", False)
+            # the barostat constructors symmetrise the domain before the integrator copies pos
+            # (npt.py:566-568, 347-349 via utils.py:346-383)
+            if self.aniso:
+                pos, rvecs = domain_symmetrize(pos, rvecs)
+            # get_ndof_baro (utils.py:478-501)
+            self.baro_ndof = (3 * 4 // 2 if self.aniso else 1) - (1 if self.volcon else 0)
         self.pos = pos
         self.rvecs = rvecs
         self.time = 0.0
@@ -192,7 +207,8 @@ class OracleMD(object):
         self.chain = None
         self.econs_thermo = 0.0
         self.econs_baro = 0.0
-        if thermo is not None:
+        set_ndof = self.ndof is not None
+        if self.thermo_kind == "nhc":
             clean_momenta(self.pos, self.vel, self.masses, self.nvec)
             if self.ndof is None:
                 self.ndof = get_ndof_internal_md(len(self.pos), self.nvec)
@@ -203,7 +219,10 @@ class OracleMD(object):
                 self.chain.vel = np.array(chain_vel0, dtype=float)
             else:
                 self.chain.set_ndof(self.ndof)
-        if self.baro is not None:
+        elif self.thermo_kind == "langevin":
+            # LangevinThermostat.init (nvt.py:193-196)
+            clean_momenta(self.pos, self.vel, self.masses, self.nvec)
+        if self.baro_kind == "mtk":
             clean_momenta(self.pos, self.vel, self.masses, self.nvec)
             if self.ndof is None:
                 self.ndof = get_ndof_internal_md(len(self.pos), self.nvec)
@@ -214,6 +233,27 @@ class OracleMD(object):
             else:
                 self.vel_press = np.array(vel_press0, dtype=float)
             self.epot, self.gpos, self.vtens = self._force(self.pos, self.rvecs)
+        elif self.baro_kind == "langevin":
+            # LangevinBarostat.init (npt.py:351-372)
+            clean_momenta(self.pos, self.vel, self.masses, self.nvec)
+            if self.ndof is None:
+                self.ndof = self.pos.size
+            timecon = self.baro.get("timecon", 1000 * femtosecond)
+            self.mass_press = (self.ndof + 3) / 3 * boltzmann * self.baro["temp"] * (
+                timecon / (2 * np.pi)) ** 2
+            if vel_press0 is None:
+                self.vel_press = get_random_vel_press(self.mass_press, self.baro["temp"])
+            else:
+                self.vel_press = np.array(vel_press0, dtype=float)
+            if self.volcon:
+                self.vel_press -= np.trace(self.vel_press) / 3 * np.eye(3)
+            if not self.aniso:
+                self.vel_press = self.vel_press[0][0]
+            self.epot, self.gpos, self.vtens = self._force(self.pos, self.rvecs)
+        # TBCombination.init (npt.py:81-95): stochastic hooks let the momentum fluctuate
+        if self.thermo is not None and self.baro is not None and not set_ndof and \
+                "langevin" in (self.thermo_kind, self.baro_kind):
+            self.ndof = self.pos.size
         if self.ndof is None:
             self.ndof = self.pos.size
         self.cons = [0, 0.0, 0.0, 0.0, 0.0]
@@ -230,12 +270,28 @@ class OracleMD(object):
     def ekin_now(self):
         return np.sum(0.5 * (self.vel ** 2.0 * self.masses.reshape(-1, 1)))
 
+    # ------------------------------------------------------------ Langevin thermostat
+    def _langevin_thermo(self, post):
+        """LangevinThermostat.pre/post (nvt.py:197-218)."""
+        th = self.thermo
+        ekin0 = self.ekin
+        c1 = np.exp(-self.timestep / th.get("timecon", 100 * femtosecond) / 2)
+        c2 = np.sqrt((1.0 - c1 ** 2) * th["temp"] * boltzmann / self.masses).reshape(-1, 1)
+        xi = thermostat_noise(th["seed"], self.counter, post, len(self.masses))
+        self.vel = c1 * self.vel + c2 * xi
+        self.ekin = self.ekin_now()
+        self.econs_thermo += ekin0 - self.ekin
+
     # ------------------------------------------------------------ MTK (npt.py:616-757)
     def _baro_ekin(self):
         return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
 
-    def _update_baro_vel(self, chainvel0):
+    def _update_baro_vel(self, chainvel0, post=0, sub=0):
         dt = self.timestep
+        lb = self.baro_kind == "langevin"
+        if lb:
+            tau = self.baro.get("timecon", 1000 * femtosecond)
+            self.vel_press *= np.exp(-dt / (8 * tau))
         if chainvel0 is not None:
             self.vel_press *= np.exp(-dt * chainvel0 / 8)
         ptens_vol = np.dot(self.vel.T * self.masses, self.vel) - self.vtens
@@ -243,53 +299,91 @@ class OracleMD(object):
         G = (ptens_vol + (2.0 * self.ekin / self.ndof
                           - self.baro["press"] * volume_of(self.rvecs)) * np.eye(3)) \
             / self.mass_press
-        self.vel_press += G * dt / 4
+        if not lb:
+            self.vel_press += G * dt / 4
+        else:
+            # LangevinBarostat.getR (npt.py:489-505) with the counter-based draw
+            rand = barostat_noise(self.baro["seed"], self.counter, post, sub) * np.sqrt(
+                2 * self.mass_press * boltzmann * self.baro["temp"] / (dt * tau))
+            R = np.zeros((3, 3))
+            for i in range(3):
+                for j in range(3):
+                    R[i, j] = rand[i, j] if i >= j else rand[j, i]
+            if self.volcon:
+                G -= np.trace(G) / self.nvec * np.eye(self.nvec)
+                R -= np.trace(R) / self.nvec * np.eye(self.nvec)
+            if not self.aniso:
+                G = np.trace(G)
+                R = R[0][0]
+            self.vel_press += (G - R / self.mass_press) * dt / 4
+            self.vel_press *= np.exp(-dt / (8 * tau))
         if chainvel0 is not None:
             self.vel_press *= np.exp(-dt * chainvel0 / 8)
 
-    def _baro(self, chainvel0):
+    def _baro(self, chainvel0, post=0):
         dt = self.timestep
-        self._update_baro_vel(chainvel0)
-        Dr, Qg = np.linalg.eigh(self.vel_press)
-        rot = np.dot(np.dot(Qg, np.diagflat(np.exp(Dr * dt / 2))), Qg.T)
-        self.pos = np.dot(self.pos, rot)
-        self.rvecs = np.dot(self.rvecs, rot)
+        epot0, ekin0 = self.epot, self.ekin
+        self._update_baro_vel(chainvel0, post, 0)
+        if self.aniso:
+            Dr, Qg = np.linalg.eigh(self.vel_press)
+            rot = np.dot(np.dot(Qg, np.diagflat(np.exp(Dr * dt / 2))), Qg.T)
+            self.pos = np.dot(self.pos, rot)
+            self.rvecs = np.dot(self.rvecs, rot)
+        else:
+            c = np.exp(self.vel_press * dt / 2)
+            self.pos = c * self.pos
+            self.rvecs = c * self.rvecs
         self.epot, self.gpos, self.vtens = self._force(self.pos, self.rvecs)
-        Dg, Eg = np.linalg.eigh(self.vel_press + (np.trace(self.vel_press) / self.ndof)
-                                * np.eye(3))
-        rot = np.dot(np.dot(Eg, np.diagflat(np.exp(-Dg * dt / 2))), Eg.T)
-        self.vel = np.dot(self.vel, rot)
+        if self.aniso:
+            if self.baro_kind == "langevin" and self.volcon:
+                Dg, Eg = np.linalg.eigh(self.vel_press)
+            else:
+                Dg, Eg = np.linalg.eigh(self.vel_press + (np.trace(self.vel_press) / self.ndof)
+                                        * np.eye(3))
+            rot = np.dot(np.dot(Eg, np.diagflat(np.exp(-Dg * dt / 2))), Eg.T)
+            self.vel = np.dot(self.vel, rot)
+        else:
+            self.vel = np.exp(-((1.0 + 3.0 / self.ndof) * self.vel_press) * dt / 2) * self.vel
         self.ekin = self.ekin_now()
-        self._update_baro_vel(chainvel0)
+        self._update_baro_vel(chainvel0, post, 1)
+        if self.baro_kind == "langevin":
+            self.econs_baro += epot0 - self.epot + ekin0 - self.ekin
 
     def _add_press_cont(self):
         return 2 * self._baro_ekin() - self.baro_ndof * (self.baro["temp"] * boltzmann)
 
     # ------------------------------------------------------------ hooks in TBC order
+    def _thermo_call(self, post):
+        if self.thermo_kind == "nhc":
+            # G1_add only from an MTK barostat (npt.py:109-113)
+            g1 = self._add_press_cont() if self.baro_kind == "mtk" else None
+            self.vel, self.ekin = self.chain(self.ekin, self.vel, g1)
+            if post:
+                self.econs_thermo = self.chain.econs_correction()
+        elif self.thermo_kind == "langevin":
+            self._langevin_thermo(post)
+
     def _pre(self):
         if self.baro is not None:
             chainvel0 = self.chain.vel[0] if self.chain is not None else None
-            self._baro(chainvel0)
-        if self.chain is not None:
-            g1 = self._add_press_cont() if self.baro is not None else None
-            self.vel, self.ekin = self.chain(self.ekin, self.vel, g1)
+            self._baro(chainvel0, 0)
+        self._thermo_call(0)
 
     def _post(self):
-        if self.chain is not None:
-            g1 = self._add_press_cont() if self.baro is not None else None
-            self.vel, self.ekin = self.chain(self.ekin, self.vel, g1)
-            self.econs_thermo = self.chain.econs_correction()
+        self._thermo_call(1)
         if self.baro is not None:
             chainvel0 = self.chain.vel[0] if self.chain is not None else None
-            self._baro(chainvel0)
-            self.econs_baro = self._baro_ekin() + self.baro["press"] * volume_of(self.rvecs)
+            self._baro(chainvel0, 1)
+            if self.baro_kind == "mtk":
+                self.econs_baro = self._baro_ekin() + self.baro["press"] * volume_of(self.rvecs)
 
     def econs_correction(self):
         corr = 0.0
-        if self.chain is not None and self.baro is not None:
-            corr = self.econs_thermo + self.econs_baro + \
-                self.baro_ndof * (boltzmann * self.chain.temp) * self.chain.pos[0]
-        elif self.chain is not None:
+        if self.thermo is not None and self.baro is not None:
+            corr = self.econs_thermo + self.econs_baro
+            if self.chain is not None:
+                corr += self.baro_ndof * (boltzmann * self.chain.temp) * self.chain.pos[0]
+        elif self.thermo is not None:
             corr = self.econs_thermo
         elif self.baro is not None:
             corr = self.econs_baro

@@ -1,0 +1,174 @@
+"""Langevin thermostat and barostat on the GPU (micmec/sampling/nvt.py:165-218,
+npt.py:295-510, combined by npt.py:52-177).
+
+Parity: the device integrator against the CPU oracle integrator fed the same counter-based
+noise (same seeds), 100 steps, every thermostat/barostat combination the reference's
+TBCombination accepts among NHC, Langevin, MTK: positions within 1e-8 Angstrom, energies,
+econs and the barostat velocity within tight relative tolerances. Statistics on the device
+alone: the sampled temperature and velocity distribution, and the sampled pressure."""
+
+import numpy as np
+import pytest
+import scipy.stats
+
+from micmec_amd import units
+from micmec_amd.pes import ForcePartMechanical, MicMecForceField
+from micmec_amd.sampling import (LangevinBarostat, LangevinThermostat, MTKBarostat,
+                                 NHCThermostat, TBCombination, VerletIntegrator, VerletScreenLog)
+from micmec_amd.workloads import make_system
+from oracle.integrators import OracleMD
+from test_md_gpu import compare, oracle_force
+
+pytestmark = pytest.mark.gpu
+
+FS = units.femtosecond
+ANG = units.angstrom
+# cell oscillation period of these nanocell systems ~ barostat timecon / 1000
+LB_TIMECON = 20000 * FS
+MTK_TIMECON = 10000 * FS
+
+
+def build(ensemble, shape=(4, 3, 5), p_b=0.3, p_hole=0.1, seed=31, model="corner8",
+          dt=1.0 * FS, press=0.2 * units.gigapascal):
+    s = make_system(shape, p_b=p_b, p_hole=p_hole, seed=seed, strain=0.005)
+    pos0 = s.pos.copy()
+    rvecs0 = np.array(s.domain.rvecs).copy()
+    mmf = MicMecForceField(s, [ForcePartMechanical(s, model=model)])
+    np.random.seed(seed)
+    hooks, th, ba = [], None, None
+    if "-lt" in ensemble:
+        hooks.append(LangevinThermostat(temp=300.0, timecon=50 * FS, seed=101))
+        th = dict(kind="langevin", temp=300.0, timecon=50 * FS, seed=101)
+    if "-nhc" in ensemble:
+        hooks.append(NHCThermostat(temp=300.0, timecon=100 * FS))
+        th = dict(temp=300.0, timecon=100 * FS, chainlength=3)
+    if "-lb" in ensemble:
+        aniso = "iso" not in ensemble
+        volcon = "volcon" in ensemble
+        hooks.append(LangevinBarostat(mmf, temp=300.0, press=press, timecon=LB_TIMECON,
+                                      anisotropic=aniso, vol_constraint=volcon, seed=202))
+        ba = dict(kind="langevin", temp=300.0, press=press, timecon=LB_TIMECON,
+                  anisotropic=aniso, vol_constraint=volcon, seed=202)
+    if "-mtk" in ensemble:
+        hooks.append(MTKBarostat(mmf, temp=300.0, press=press, timecon=MTK_TIMECON))
+        ba = dict(temp=300.0, press=press, timecon=MTK_TIMECON)
+    verlet = VerletIntegrator(mmf, timestep=dt, hooks=hooks, temp0=300.0)
+    np.random.seed(seed)
+    omd = OracleMD(oracle_force(s, model), pos0, rvecs0, s.masses, dt, temp0=300.0,
+                   thermo=th, baro=ba)
+    return s, verlet, omd
+
+
+def hooks_of(verlet):
+    thermo = baro = None
+    for h in verlet.hooks:
+        if isinstance(h, TBCombination):
+            thermo, baro = h.thermostat, h.barostat
+        elif isinstance(h, (LangevinThermostat, NHCThermostat)):
+            thermo = h
+        elif isinstance(h, (LangevinBarostat, MTKBarostat)):
+            baro = h
+    return thermo, baro
+
+
+@pytest.mark.parametrize("ensemble,model", [
+    ("nvt-lt", "corner8"), ("nvt-lt", "original"), ("npt-lt-lb", "corner8"),
+    ("npt-lt-lb", "original"), ("npt-lt-lb-iso", "corner8"), ("npt-lt-lb-volcon", "corner8"),
+    ("npt-nhc-lb", "corner8"), ("npt-lt-mtk", "corner8"), ("nph-lb", "corner8")])
+def test_langevin_trajectory_parity(ensemble, model):
+    print("langevin parity", ensemble, model, flush=True)
+    s, verlet, omd = build(ensemble, model=model)
+    compare(verlet, omd, "init")
+    assert verlet.ndof == omd.ndof
+    for chunk in range(4):
+        verlet.run(25)
+        omd.run(25)
+        compare(verlet, omd, "%s step %d" % (ensemble, verlet.counter))
+    thermo, baro = hooks_of(verlet)
+    if thermo is not None:
+        ref = omd.econs_thermo
+        assert abs(thermo.econs_correction - ref) <= 1e-8 * abs(ref) + 1e-14
+    if baro is not None:
+        vp, ref = np.asarray(baro.vel_press), np.asarray(omd.vel_press)
+        assert np.abs(vp - ref).max() <= 1e-8 * np.abs(ref).max()
+        assert np.abs(verlet.rvecs - omd.rvecs).max() / ANG < 1e-8
+        ref = omd.econs_baro
+        assert abs(baro.econs_correction - ref) <= 1e-8 * abs(ref) + 1e-14
+
+
+def test_graph_and_eager_agree_bitwise():
+    _, v1, _ = build("npt-lt-lb", seed=5)
+    _, v2, _ = build("npt-lt-lb", seed=5)
+    v2.use_graph = False
+    v1.run(30)
+    v2.run(30)
+    assert np.array_equal(v1.pos, v2.pos) and np.array_equal(v1.vel, v2.vel)
+    assert v1.econs == v2.econs
+
+
+def test_langevin_thermostat_samples_the_canonical_velocities():
+    s = make_system((8, 8, 8), p_b=0.2, p_hole=0.05, seed=4)
+    mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+    np.random.seed(4)
+    temp = 400.0
+    lt = LangevinThermostat(temp=temp, timecon=40 * FS, seed=9)
+    verlet = VerletIntegrator(mmf, timestep=2.0 * FS, hooks=[lt], temp0=100.0)
+    verlet.run(400)  # 20 time constants
+    temps, scaled = [], []
+    for k in range(30):
+        verlet.run(40)  # one time constant apart: nearly independent samples
+        temps.append(verlet.temp)
+        scaled.append((verlet.vel * np.sqrt(s.masses[:, None] / (units.boltzmann * temp))).ravel())
+    temps = np.array(temps)
+    # relative spread of one sample sqrt(2 / ndof) ~ 2 %; mean of 30 samples ~ 0.4 %
+    assert abs(temps.mean() / temp - 1.0) < 0.02, temps.mean()
+    z = np.concatenate(scaled[::6])
+    assert scipy.stats.kstest(z, "norm").pvalue > 1e-4
+    assert abs(z.var() - 1.0) < 0.05
+
+
+def test_langevin_barostat_samples_the_target_pressure():
+    s = make_system((6, 6, 6), p_b=0.0, p_hole=0.0, seed=12, strain=0.02)
+    mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+    np.random.seed(12)
+    P = 1.0 * units.gigapascal
+    lt = LangevinThermostat(temp=300.0, timecon=100 * FS, seed=21)
+    lb = LangevinBarostat(mmf, temp=300.0, press=P, timecon=2 * LB_TIMECON, seed=22)
+    verlet = VerletIntegrator(mmf, timestep=2.0 * FS, hooks=[lt, lb], temp0=300.0)
+    p_start = verlet.press
+    verlet.run(1000)
+    samples, offdiag = [], []
+    for _ in range(3000):
+        verlet.propagate()
+        samples.append(verlet.press)
+        pt = verlet.ptens
+        offdiag.append([pt[0, 1], pt[0, 2], pt[1, 2]])
+    samples = np.array(samples)
+    blocks = samples.reshape(10, -1).mean(axis=1)
+    err = blocks.std(ddof=1) / np.sqrt(len(blocks))
+    assert abs(p_start - P) > 0.5 * P
+    assert abs(samples.mean() - P) < max(5 * err, 0.05 * P), (samples.mean() / P, err / P)
+    assert np.abs(np.mean(offdiag, axis=0)).max() < 0.05 * P
+
+
+def test_md_script_flow_runs_with_langevin_hooks():
+    """The NVT and NPT branches of simulations/md.py:48-66: Langevin thermostat (timecon
+    100 dt) and Langevin barostat (timecon 1000 dt) merged by TBCombination, screen log."""
+    for press in (None, 10.0 * units.bar):
+        s = make_system((4, 4, 4), p_b=0.2, p_hole=0.0, seed=2)
+        mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+        np.random.seed(1)
+        dt = 1.0 * FS
+        temp = 300.0
+        hooks = [VerletScreenLog(step=10)]
+        lt = LangevinThermostat(temp=temp, timecon=100 * dt)
+        if press is None:
+            hooks.append(lt)
+        else:
+            lb = LangevinBarostat(mmf, temp=temp, press=press, timecon=20000 * dt)
+            hooks.append(TBCombination(lt, lb))
+        verlet = VerletIntegrator(mmf, timestep=dt, temp0=temp, hooks=hooks)
+        verlet.run(200)
+        assert np.isfinite(verlet.econs) and np.isfinite(verlet.pos).all()
+        assert 100.0 < verlet.temp < 600.0
+        assert lt.seed is not None  # drawn from numpy's stream at initialisation
