@@ -8,6 +8,7 @@ from .nvt import NHChain, NHCThermostat, LangevinThermostat, NHCAttributeStateIt
 from .npt import TBCombination, MTKBarostat, LangevinBarostat, MTKAttributeStateItem
 from .dof import DOF, CartesianDOF, BaseCellDOF, FullCellDOF, StrainCellDOF
 from .opt import OptScreenLog, BaseOptimizer, CGOptimizer, ConjugateGradientSearch
+from .trajectory import BaseHDF5Writer, HDF5Writer, XYZWriter
 
 __all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
            "TemperatureStateItem", "VolumeStateItem", "DomainStateItem", "ConsErrStateItem",
@@ -16,4 +17,5 @@ __all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
            "NHCAttributeStateItem", "TBCombination", "MTKBarostat", "LangevinBarostat",
            "MTKAttributeStateItem", "DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF",
            "StrainCellDOF",
-           "OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch"]
+           "OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch",
+           "BaseHDF5Writer", "HDF5Writer", "XYZWriter"]

@@ -1,7 +1,8 @@
 """The micromechanical system container and its periodic domain.
 
-``System`` mirrors micmec/system.py:38-350: the same constructor arguments, ``from_file`` and
-``to_file`` for CHK files, and attribute names read by the force field and integrators.
+``System`` mirrors micmec/system.py:38-370: the same constructor arguments, ``from_file`` and
+``to_file`` for CHK, HDF5 (where h5py is installed) and XYZ files, ``to_hdf5`` / ``from_hdf5``
+on open h5py-style files, and attribute names read by the force field and integrators.
 ``Domain`` mirrors the Cython extension type of micmec/pes/ext.pyx:36-123 and its C helper
 micmec/pes/domain.c:13-49 (rvecs, gvecs from the SVD pseudo-inverse, volume, nvec).
 """
@@ -103,6 +104,14 @@ class Domain(object):
         return lengths, np.arccos(np.clip(cosines, -1, 1))
 
 
+def _import_h5py():
+    try:
+        import h5py
+    except ImportError as exc:
+        raise ImportError("HDF5 files need h5py, which is not installed") from exc
+    return h5py
+
+
 class System(object):
     """A micromechanical system (system.py:38-107); all quantities in atomic units."""
 
@@ -124,9 +133,16 @@ class System(object):
         for key, val in new_params.items():
             self.params["type%i/%s" % (int(type_), key)] = val
 
+    _HDF5_KEYS = ("pos", "masses", "rvecs", "surrounding_cells", "surrounding_nodes",
+                  "boundary_nodes", "grid", "types")
+
     @classmethod
     def from_file(cls, fn, **user_kwargs):
-        """Construct a system from a CHK file (system.py:213-259)."""
+        """Construct a system from a CHK or HDF5 file (system.py:213-259)."""
+        if fn.endswith(".h5"):
+            h5py = _import_h5py()
+            with h5py.File(fn, "r") as f:
+                return cls.from_hdf5(f, **user_kwargs)
         if not fn.endswith(".chk"):
             raise IOError('Cannot read from file "%s".' % fn)
         allowed = ["pos", "masses", "rvecs", "surrounding_cells", "surrounding_nodes",
@@ -157,10 +173,62 @@ class System(object):
             params=params,
         )
 
+    @classmethod
+    def from_hdf5(cls, f, **user_kwargs):
+        """Construct a system from the ``system`` group of an open HDF5 file (system.py:261-296).
+        Type parameters stored as ``params/<name>`` datasets are read back as well."""
+        sgrp = f["system"]
+        kwargs = {"pos": np.asarray(sgrp["pos"][:])}
+        for key in cls._HDF5_KEYS:
+            if key in sgrp:
+                kwargs[key] = np.asarray(sgrp[key][:])
+        if "params" in sgrp:
+            pgrp = sgrp["params"]
+            kwargs["params"] = {k.replace(":", "/"): np.asarray(pgrp[k][()])
+                                for k in pgrp.keys()}
+        kwargs.update(user_kwargs)
+        for key in ("surrounding_cells", "surrounding_nodes"):
+            kwargs.setdefault(key, np.zeros((0, 8), dtype=int))
+        kwargs.setdefault("masses", None)
+        kwargs.setdefault("rvecs", np.zeros((0, 3)))
+        return cls(**kwargs)
+
+    def to_hdf5(self, f):
+        """Write the system to the ``system`` group of an open HDF5 file (system.py:344-370).
+
+        As the reference: positions and masses, and a ValueError if the file already holds a
+        system. In addition the domain, topology and type parameters (under ``params``, with
+        ':' for '/') are stored, so that ``from_hdf5`` gives back the whole system."""
+        if "system" in f:
+            raise ValueError("The HDF5 file already contains a system description.")
+        sgrp = f.create_group("system")
+        sgrp.create_dataset("pos", data=np.asarray(self.pos))
+        if self.masses is not None:
+            sgrp.create_dataset("masses", data=np.asarray(self.masses))
+        for key in self._HDF5_KEYS[2:]:
+            value = np.asarray(self.domain.rvecs) if key == "rvecs" else getattr(self, key)
+            if value is not None:
+                sgrp.create_dataset(key, data=np.asarray(value))
+        if self.params:
+            pgrp = sgrp.create_group("params")
+            for k, v in self.params.items():
+                pgrp.create_dataset(k.replace("/", ":"), data=np.asarray(v))
+
     def to_file(self, fn):
-        """Write the system to a CHK file (system.py:323-335)."""
+        """Write the system to a CHK, HDF5 or XYZ file (system.py:298-342)."""
+        if fn.endswith(".h5"):
+            h5py = _import_h5py()
+            with h5py.File(fn, "w") as f:
+                self.to_hdf5(f)
+            return
+        if fn.endswith(".xyz"):
+            from .sampling.trajectory import write_xyz_frame
+            with open(fn, "w") as fh:
+                write_xyz_frame(fh, ["Cs"] * len(self.pos), str(self), self.pos)
+            return
         if not fn.endswith(".chk"):
-            raise NotImplementedError("Only the CHK format is supported: %s" % fn)
+            raise NotImplementedError(
+                "The extension of %s does not correspond to any known format." % fn)
         output = {
             "pos": np.asarray(self.pos),
             "masses": np.asarray(self.masses),

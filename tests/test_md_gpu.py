@@ -159,3 +159,24 @@ def test_nan_in_md_raises():
     verlet._engine.pos[3, 0] = float("nan")
     with pytest.raises(ValueError):
         verlet.run(2)
+
+
+def test_hdf5_writer_records_md_frames():
+    """HDF5Writer on the device integrator (trajectory.py:31-87): one row per due step, every
+    state item, the values the integrator holds at that step."""
+    from h5mem import Group
+    from micmec_amd.sampling import HDF5Writer
+
+    s, verlet, omd = build((3, 3, 4), 0.3, 0.0, 17, "npt-nhc-mtk", "corner8")
+    f = Group()
+    verlet.hooks.append(HDF5Writer(f, step=10))
+    verlet.run(40)
+    omd.run(40)
+    traj = f["trajectory"]
+    assert list(traj["counter"][:]) == [10, 20, 30, 40]
+    assert np.array_equal(traj["pos"][3], verlet.pos)
+    assert traj["epot"][3] == verlet.epot and traj["econs"][3] == verlet.econs
+    assert np.abs(traj["pos"][3] - omd.pos).max() / ANG < 1e-8
+    assert np.array_equal(traj["domain"][3], verlet.rvecs)
+    assert traj["temp"].shape == (4,) and traj["ptens"].shape == (4, 3, 3)
+    assert np.array_equal(f["system"]["masses"][:], s.masses)
