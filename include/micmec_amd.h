@@ -98,23 +98,6 @@ int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, v
 /* Sizes of the object: cells, nodes, padded work slots. */
 int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* nslots);
 
-/* Spatial tiling of the cell pass. cell_ijk [ncells][3] holds the grid point of every cell (in
- * the order the object was created with), sys_dims [nsys][3] the cells of every system's grid
- * along x, y, z and sys_pbc [nsys] whether it is periodic (its wrap bits must then be those of
- * the grid's last layers, as micmec/pes/mmff.py:258-286 implies). After this call every force
- * evaluation of the object runs one workgroup per tile of tx * ty * tz cells: node positions
- * staged in LDS, vertex gradients summed per node slot in LDS, then one node pass; results
- * equal those of the untiled kernels to rounding. The topology is checked against the grid;
- * on MM_ERR_ARG (inconsistent grid, or a tile whose cells need more than 1024 lanes or 64 KB of
- * LDS) the object keeps its untiled kernels. Replaces nothing in the reference: the
- * reference evaluates cells with numpy (mmff.py:326-403). */
-int mm_field_set_grid(mm_field* f, const int32_t* cell_ijk, const int32_t* sys_dims,
-                      const int32_t* sys_pbc, int32_t tx, int32_t ty, int32_t tz);
-
-/* Tiling state: enabled flag, number of tiles, lanes (threads) per tile, node-slot entries. */
-int mm_field_tiles_info(const mm_field* f, int32_t* enabled, int64_t* ntiles, int32_t* lanes,
-                        int64_t* entries);
-
 void mm_field_destroy(mm_field* f);
 
 /* Replace the cell-type parameters of an existing object (same number of rows and at most

@@ -51,9 +51,6 @@ def _load():
     L.mm_field_launch_cells.argtypes = [P, P, P, P]
     L.mm_field_info.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.mm_field_destroy.argtypes = [P]
-    L.mm_field_set_grid.argtypes = [P, P, P, P, i32, i32, i32]
-    L.mm_field_tiles_info.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i64),
-                                      ctypes.POINTER(i32), ctypes.POINTER(i64)]
     L.mm_field_destroy.restype = None
     L.mm_md_destroy.argtypes = [P]
     L.mm_md_destroy.restype = None

@@ -44,7 +44,6 @@ from torch.autograd.function import once_differentiable
 
 from . import _native, units
 from .batch import concat_systems
-from .pes.mmff import enable_tiles, grid_tables
 
 __all__ = ["DiffField", "CellParams", "micmec_energy", "voigt_from_tensor", "tensor_from_voigt",
            "voigt_pairs", "fit_workload", "FitProblem", "fit"]
@@ -129,7 +128,7 @@ class DiffField(object):
     array; ``node_sys`` [N] gives each row's system (padding rows: the preceding system, and
     they carry no cell)."""
 
-    def __init__(self, systems, model="corner8", device=None, tiles=True):
+    def __init__(self, systems, model="corner8", device=None):
         if model not in _native.MODELS:
             raise ValueError("Unknown model %r; choose one of %s" % (model, list(_native.MODELS)))
         self.systems = list(systems)
@@ -158,10 +157,6 @@ class DiffField(object):
             float(units.boltzmann))
         _native.check(rc, "mm_batch_field_create")
         self._field = handle
-        self.tile_shape = None
-        if tiles:
-            pbcs = [np.asarray(s.domain.rvecs).shape[0] > 0 for s in self.systems]
-            self.tile_shape = enable_tiles(self.lib, handle, grid_tables(self.systems, pbcs))
         adj = ctypes.c_void_p()
         npar = _i64(0)
         rc = self.lib.mm_adjoint_create(ctypes.byref(adj), self._field, ctypes.byref(npar))

@@ -21,7 +21,7 @@ import torch
 
 from . import _layout as L
 from . import _native, units
-from .pes.mmff import enable_tiles, grid_tables, parse_type_params, system_wrap_bits
+from .pes.mmff import parse_type_params, system_wrap_bits
 from .sampling.utils import clean_momenta, get_ndof_internal_md, get_random_vel
 
 __all__ = ["BatchVerlet", "NODE_ALIGN", "concat_systems"]
@@ -122,13 +122,10 @@ class BatchVerlet(object):
         numpy legacy-stream seed of every system's random initial velocities.
     thermostat : bool
         Nose-Hoover chain (length ``chainlength``, time constant ``timecon``) on every system.
-    tiles : bool
-        Tiled cell pass (see ``ForcePartMechanical``) when the systems' grids allow it.
     """
 
     def __init__(self, systems, timestep, temps, seeds, model="corner8", thermostat=True,
-                 timecon=100 * units.femtosecond, chainlength=3, device=None, use_graph=True,
-                 tiles=True):
+                 timecon=100 * units.femtosecond, chainlength=3, device=None, use_graph=True):
         self.systems = list(systems)
         self.nsys = n = len(self.systems)
         if n < 1:
@@ -157,10 +154,6 @@ class BatchVerlet(object):
             int(h0.shape[1]), vp(ns), vp(h0), vp(c0), vp(free), vp(temp), float(units.boltzmann))
         _native.check(rc, "mm_batch_field_create")
         self.field = handle
-        self.tile_shape = None
-        if tiles:
-            pbcs = [np.asarray(s.domain.rvecs).shape[0] > 0 for s in self.systems]
-            self.tile_shape = enable_tiles(self.lib, handle, grid_tables(self.systems, pbcs))
         # ---------------------------------------------------------- host initialisation
         pos = np.zeros((Ntot, 3))
         vel = np.zeros((Ntot, 3))

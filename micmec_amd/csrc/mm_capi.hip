@@ -164,7 +164,6 @@ void free_field(mm_field* f) {
   }
   if (f->ev_fork) (void)hipEventDestroy(f->ev_fork);
   if (f->ev_join) (void)hipEventDestroy(f->ev_join);
-  free_tiles(f->tiles);
   (void)hipSetDevice(cur);
   delete f;
 }
@@ -395,16 +394,6 @@ int mm_batch_compute(mm_field* f, const double* pos, const double* rvecs, double
   if ((f->N > 0 && !pos) || !rvecs || !energies)
     return set_error(MM_ERR_ARG, "pos, rvecs and energies are required");
   DeviceGuard guard(f->device);
-  if (f->tiles.enabled) {
-    hipError_t e = launch_tile_cells(f, pos, rvecs, 9, true, (hipStream_t)stream);
-    if (e == hipSuccess && gpos) e = launch_tile_gather(f, gpos, 0, (hipStream_t)stream);
-    if (e == hipSuccess)
-      e = launch_batch_finalize(f->tiles.partials, f->tiles.ntiles, f->tiles.sys_ptr,
-                                f->tiles.sys_idx, f->nsys, energies, vtens, f->nanflag,
-                                (hipStream_t)stream);
-    if (e != hipSuccess) return hip_error(e, "batched force kernels");
-    return MM_OK;
-  }
   ForceArgs a;
   a.pos = pos;
   a.rvecs = rvecs;
@@ -447,16 +436,6 @@ int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double
   if (f->N > 0 && !pos) return set_error(MM_ERR_ARG, "pos is NULL");
   if (!rvecs) return set_error(MM_ERR_ARG, "rvecs is NULL");
   DeviceGuard guard(f->device);
-  if (f->tiles.enabled) {
-    const bool grad = gpos != nullptr || vtens != nullptr;
-    hipError_t e = launch_tile_cells(f, pos, rvecs, 0, grad, (hipStream_t)stream);
-    if (e == hipSuccess && gpos) e = launch_tile_gather(f, gpos, accumulate, (hipStream_t)stream);
-    if (e == hipSuccess)
-      e = launch_finalize(f->tiles.partials, f->tiles.ntiles, energy, vtens, accumulate,
-                          f->nanflag, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_error(e, "force kernels");
-    return MM_OK;
-  }
   ForceArgs a;
   a.pos = pos;
   a.rvecs = rvecs;
@@ -507,11 +486,6 @@ int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, v
   if (!f) return set_error(MM_ERR_ARG, "field is NULL");
   if ((f->N > 0 && !pos) || !rvecs) return set_error(MM_ERR_ARG, "pos or rvecs is NULL");
   DeviceGuard guard(f->device);
-  if (f->tiles.enabled) {
-    hipError_t e = launch_tile_cells(f, pos, rvecs, 0, true, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_error(e, "tile kernel");
-    return MM_OK;
-  }
   ForceArgs a;
   a.pos = pos;
   a.rvecs = rvecs;

@@ -153,16 +153,6 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
   }
 }
 
-// Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
-// every XCD has its own L2. Neighbouring segments share nodes, so each XCD is given one
-// contiguous range of the n blocks: block b -> (b mod 8) * (n / 8) + min(b mod 8, n mod 8) + b / 8.
-// The map is a bijection of [0, n) for any n.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block(int b, int n) {
-  const int q = n / kXcds, r = n % kXcds, x = b % kXcds, i = b / kXcds;
-  return x * q + (x < r ? x : r) + i;
-}
-
 template <int NV>
 __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
 #pragma unroll

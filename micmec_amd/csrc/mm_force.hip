@@ -69,6 +69,16 @@ __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
   }
 }
 
+// Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
+// every XCD has its own L2. Neighbouring segments share nodes, so each XCD is given one
+// contiguous range of the n blocks: block b -> (b mod 8) * (n / 8) + min(b mod 8, n mod 8) + b / 8.
+// The map is a bijection of [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n / kXcds, r = n % kXcds, x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // One lane per cell. Energies of all states of the block's cell type, combined by the
 // log-sum-exp of mmff.py:386-398; the vertex gradient of the combined energy goes to gcell and
 // the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials. A block holds
@@ -255,13 +265,6 @@ hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {
   const int gb = (int)((a.N + 255) / 256);
   hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N, a.W,
                      a.gpos, a.accumulate, a.nanflag);
-  return hipGetLastError();
-}
-
-hipError_t launch_finalize(const double* partials, int nb, double* energy, double* vtens,
-                           int accumulate, int* nanflag, hipStream_t stream) {
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, stream, partials, nb, energy,
-                     vtens, accumulate, nanflag);
   return hipGetLastError();
 }
 

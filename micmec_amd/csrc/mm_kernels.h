@@ -46,9 +46,6 @@ hipError_t launch_batch_finalize(const double* partials, int nb, const int32_t* 
                                  double* vtens, int* nanflag, hipStream_t stream);
 // Node gather of gcell into gpos (overwrite or accumulate, as a.accumulate).
 hipError_t launch_gather(const ForceArgs& a, hipStream_t stream);
-// Energy and virial totals of nb block partials (overwrite or accumulate).
-hipError_t launch_finalize(const double* partials, int nb, double* energy, double* vtens,
-                           int accumulate, int* nanflag, hipStream_t stream);
 // Cell kernel only: per-cell gradients into gcell, energy/virial block partials.
 hipError_t launch_cells(const ForceArgs& a, hipStream_t stream);
 

@@ -230,16 +230,12 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities; with kLT also
 // the Langevin thermostat's post update (the node sums then hold the ekin before it in [13]).
 // One node per thread (the gathers need every wave in flight that they can get).
-// kCSR: the node sums come from the slot sums of the tiled cell pass (hptr/hent over nodeacc)
-// instead of the per-cell vertex gradients (ncell over gcell).
-template <bool kKick, bool kLT, bool kCSR>
+template <bool kKick, bool kLT>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t W, double* __restrict__ gpos, double* __restrict__ v,
                    const double* __restrict__ m, const double* __restrict__ S0,
-                   const int32_t* __restrict__ blk_sys, double* __restrict__ partials,
-                   const int32_t* __restrict__ hptr, const int32_t* __restrict__ hent,
-                   const double* __restrict__ nodeacc) {
+                   const int32_t* __restrict__ blk_sys, double* __restrict__ partials) {
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
   constexpr int NA = kLT ? 14 : 13;
   double acc[NA];
@@ -249,23 +245,14 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
     if (n < N) {
     double g[3] = {0.0, 0.0, 0.0};
-    if (kCSR) {
-      const int e1 = hptr[n + 1];
-      for (int e = hptr[n]; e < e1; ++e) {
-        const int64_t idx = 3 * (int64_t)hent[e];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) g[j] += nodeacc[idx + j];
-      }
-    } else {
+    for (int k = 0; k < 8; ++k) {
+      const int32_t e = ncell[k * N + n];
+      if (e >= 0) {
+        const int64_t w = e >> 3;
+        const int kk = e & 7;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int32_t e = ncell[k * N + n];
-        if (e >= 0) {
-          const int64_t w = e >> 3;
-          const int kk = e & 7;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
-        }
+        for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
       }
     }
 #pragma unroll
@@ -1016,15 +1003,12 @@ bool reduces(int phase) {
 }
 
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
-  const mm_field* f = md->field;
-  const bool tiled = f->tiles.enabled != 0;
-  const int nbf = tiled ? f->tiles.ntiles : (int)(f->W / kCellBlock);
+  const int nbf = (int)(md->field->W / kCellBlock);
   const int G = reduces(phase) ? md->slices : 1;
   hipLaunchKernelGGL(scalar_kernel, dim3(G, md->nsys), dim3(kScalarThreads), 0, s, phase, md->S,
-                     tiled ? f->tiles.partials : f->partials, nbf,
-                     tiled ? f->tiles.sys_ptr : f->sys_cblk_ptr,
-                     tiled ? f->tiles.sys_idx : f->sys_cblk_idx, md->npart, md->nbn,
-                     md->sys_nblk, with_vtens, md->slice_buf, md->counters);
+                     md->field->partials, nbf, md->field->sys_cblk_ptr,
+                     md->field->sys_cblk_idx, md->npart, md->nbn, md->sys_nblk, with_vtens,
+                     md->slice_buf, md->counters);
   return hipGetLastError();
 }
 
@@ -1032,31 +1016,19 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
 // thermostat's post update when lt)
 hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s) {
   ForceArgs a = force_args(md);
-  const mm_tiles& tl = md->field->tiles;
-  const bool tiled = tl.enabled != 0;
-  hipError_t e = tiled ? launch_tile_cells(md->field, md->pos, md->S + MM_S_RVECS, MM_S_COUNT,
-                                           true, s)
-                       : launch_cells(a, s);
+  hipError_t e = launch_cells(a, s);
   if (e != hipSuccess) return e;
   if (md->N > 0) {
-#define MM_GATHER(K, T, C)                                                                   \
-  hipLaunchKernelGGL((gather_kick_kernel<K, T, C>), dim3(md->nbn), dim3(kNodeBlock), 0, s,    \
+#define MM_GATHER(K, T)                                                                      \
+  hipLaunchKernelGGL((gather_kick_kernel<K, T>), dim3(md->nbn), dim3(kNodeBlock), 0, s,       \
                      a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,      \
-                     md->node_blk_sys, md->npart, tl.hptr, tl.hent, tl.nodeacc)
-#define MM_GATHER_ALL(C)          \
-  do {                            \
-    if (kick && lt)               \
-      MM_GATHER(true, true, C);   \
-    else if (kick)                \
-      MM_GATHER(true, false, C);  \
-    else                          \
-      MM_GATHER(false, false, C); \
-  } while (0)
-    if (tiled)
-      MM_GATHER_ALL(true);
+                     md->node_blk_sys, md->npart)
+    if (kick && lt)
+      MM_GATHER(true, true);
+    else if (kick)
+      MM_GATHER(true, false);
     else
-      MM_GATHER_ALL(false);
-#undef MM_GATHER_ALL
+      MM_GATHER(false, false);
 #undef MM_GATHER
     e = hipGetLastError();
   }

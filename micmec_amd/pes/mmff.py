@@ -200,48 +200,6 @@ def parse_type_params(params, types):
     return type_ids, np.array(ns, dtype=np.int32), h0, c0, free, temp
 
 
-# Tile shapes tried in turn by enable_tiles: the largest whose cells fit one workgroup.
-TILE_SHAPES = ((8, 8, 8), (8, 8, 4), (8, 4, 4), (4, 4, 4), (4, 4, 2), (4, 2, 2), (2, 2, 2))
-
-
-def grid_tables(systems, pbcs):
-    """Cell grid points [C][3] (systems concatenated in order), cells per axis [nsys][3] and
-    periodicity [nsys] of systems whose cells follow the order of their ``grid``
-    (non-empty grid points, x-major, z fastest); None if a system has no such grid."""
-    ijk, dims = [], []
-    for s in systems:
-        g = getattr(s, "grid", None)
-        if g is None:
-            return None
-        g = np.rint(np.asarray(g, dtype=np.float64)).astype(np.int64)
-        if g.ndim != 3:
-            return None
-        cells = np.argwhere(g != 0)
-        if len(cells) != s.ncells:
-            return None
-        ijk.append(cells)
-        dims.append(g.shape)
-    cell_ijk = np.ascontiguousarray(np.concatenate(ijk) if ijk else np.zeros((0, 3)),
-                                    dtype=np.int32)
-    return (cell_ijk, np.ascontiguousarray(dims, dtype=np.int32),
-            np.ascontiguousarray([1 if p else 0 for p in pbcs], dtype=np.int32))
-
-
-def enable_tiles(lib, handle, tables, shapes=TILE_SHAPES):
-    """Switch an ``mm_field`` to the tiled cell pass (mm_field_set_grid), trying the tile shapes
-    in turn. Returns the shape used, or None when the grid does not describe the topology (the
-    object then keeps its untiled kernels)."""
-    if tables is None:
-        return None
-    cell_ijk, dims, pbc = tables
-    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    for shape in shapes:
-        rc = lib.mm_field_set_grid(handle, vp(cell_ijk), vp(dims), vp(pbc), *shape)
-        if rc == 0:
-            return shape
-    return None
-
-
 class DeviceField(object):
     """Owner of one ``mm_field`` object of the HIP library (device topology and parameters)."""
 
@@ -274,12 +232,6 @@ class DeviceField(object):
         _native.check(rc, "mm_field_create")
         self._handle = handle
         self._lib = L
-        self.tile_shape = None
-
-    def set_grid(self, tables):
-        """Use the tiled cell pass (see ``enable_tiles``); returns the tile shape or None."""
-        self.tile_shape = enable_tiles(self._lib, self._handle, tables)
-        return self.tile_shape
 
     def compute(self, pos, rvecs, energy, gpos=None, vtens=None, accumulate=False, stream=None):
         """Launch the force kernels (asynchronous): all arguments are float64 device tensors."""
@@ -338,13 +290,9 @@ class ForcePartMechanical(ForcePart):
         Nanocell energy model; see the module docstring.
     device : torch device, optional
         GPU that holds the topology and parameter tables (default: the current device).
-    tiles : bool, optional
-        Evaluate the cells in spatial tiles (one workgroup per box of cells, node sums in LDS;
-        the default) when the system's ``grid`` describes its topology; ``False`` keeps one
-        lane per cell with vertex gradients in HBM. Both give the same results to rounding.
     """
 
-    def __init__(self, system, model="corner8", device=None, tiles=True):
+    def __init__(self, system, model="corner8", device=None):
         ForcePart.__init__(self, "micmec", system)
         self.system = system
         self.model = model
@@ -358,8 +306,6 @@ class ForcePartMechanical(ForcePart):
         self.type_ids = type_ids
         self.field = DeviceField(system.surrounding_nodes, system.surrounding_cells, wrap,
                                  cell_row, ns, h0, c0, free, temp, model=model, device=device)
-        if tiles:
-            self.field.set_grid(grid_tables([system], [self.pbc]))
         self.device = self.field.device
         N = system.nnodes
         kw = dict(dtype=torch.float64, device=self.device)
