@@ -226,6 +226,30 @@ void mm_adjoint_destroy(mm_adjoint* a);
 #define MM_S_BARO_EKIN0 144
 #define MM_S_COUNT 160
 
+/* ------------------------------------------------------------------------------------------
+ * Topology of a grid of cells, built on the device: Grid + build_system (micmec/utils.py:55-263)
+ * for grids too large for a host pass. grid [nx][ny][nz] (host, x-major) holds the type of every
+ * grid point (0 = empty, 1..ntypes); pbc_bits: bit d = periodic along axis d; type_mass
+ * [ntypes + 1] the mass of every type (entry 0 unused); spacing [3] the lattice spacing
+ * (the diagonal of the majority type's first cell matrix, utils.py:220-233). The tables equal
+ * build_system's: cells and nodes in its order, corner k of a cell at offset
+ * (000,100,010,001,110,101,011,111), -1 for missing cells, node masses summed in (cell, corner)
+ * order, plus the wrap bits of the force kernels and the grid index of every cell.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct mm_topology mm_topology;
+
+int mm_topology_build(mm_topology** out, int device, const int32_t* grid, int32_t nx,
+                      int32_t ny, int32_t nz, int32_t pbc_bits, const double* type_mass,
+                      int32_t ntypes, const double* spacing, void* stream);
+int mm_topology_sizes(const mm_topology* t, int64_t* ncells, int64_t* nnodes, int64_t* nboundary);
+/* Copy the tables out (any pointer may be NULL to skip that table): surrounding_nodes [C][8],
+ * surrounding_cells [N][8], boundary_nodes [B], masses [N], pos [N][3], types [C],
+ * wrap_bits [C], cell_point [C] (linear grid index). Host or device destinations. */
+int mm_topology_copy(const mm_topology* t, int64_t* surrounding_nodes, int64_t* surrounding_cells,
+                     int64_t* boundary_nodes, double* masses, double* pos, int32_t* types,
+                     uint8_t* wrap_bits, int32_t* cell_point, void* stream);
+void mm_topology_destroy(mm_topology* t);
+
 typedef struct mm_md mm_md;
 
 /* Bind an integrator to a force field and to caller-owned device buffers. thermo/baro select

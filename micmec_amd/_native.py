@@ -51,6 +51,12 @@ def _load():
     L.mm_field_launch_cells.argtypes = [P, P, P, P]
     L.mm_field_info.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.mm_field_destroy.argtypes = [P]
+    L.mm_topology_build.argtypes = [ctypes.POINTER(P), ctypes.c_int, P, i32, i32, i32, i32, P,
+                                    i32, P, P]
+    L.mm_topology_sizes.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                    ctypes.POINTER(i64)]
+    L.mm_topology_copy.argtypes = [P, P, P, P, P, P, P, P, P, P]
+    L.mm_topology_destroy.argtypes = [P]
     L.mm_field_destroy.restype = None
     L.mm_md_destroy.argtypes = [P]
     L.mm_md_destroy.restype = None

@@ -25,8 +25,8 @@ import numpy as np
 from . import units
 
 __all__ = [
-    "NEIGHBOR_CELLS", "NEIGHBOR_NODES", "Grid", "build_system", "build_type",
-    "periodic_wrap_bits",
+    "NEIGHBOR_CELLS", "NEIGHBOR_NODES", "Grid", "build_system", "build_system_device",
+    "build_type", "periodic_wrap_bits",
 ]
 
 # Offsets of the eight cells around a node; slot k holds the cell in which the node is corner k
@@ -198,9 +198,7 @@ def build_system(data, grid, pbc=None):
     contrib = np.repeat(0.125 * mass_of[types], 8)
     np.add.at(masses, sn.ravel(), contrib)
     # Lattice positions from the majority type's first equilibrium cell.
-    counts = np.bincount(np.rint(grid).astype(np.int64).ravel())
-    maj = int(np.argmax(counts[1:]) + 1)
-    dx, dy, dz = np.diag(np.asarray(data[maj]["cell"][0], dtype=np.float64))
+    dx, dy, dz = _majority_spacing(data, grid)
     ijk = gs.node_ijk
     pos = np.zeros((N, 3))
     pos += np.stack([ijk[:, 0] * dx, ijk[:, 1] * dy, ijk[:, 2] * dz], axis=1)
@@ -226,6 +224,93 @@ def build_system(data, grid, pbc=None):
     for type_idx, type_data in data.items():
         for key, value in type_data.items():
             output["type%i/%s" % (type_idx, key)] = value
+    return output
+
+
+def _majority_spacing(data, grid):
+    counts = np.bincount(np.rint(grid).astype(np.int64).ravel())
+    maj = int(np.argmax(counts[1:]) + 1)
+    return np.diag(np.asarray(data[maj]["cell"][0], dtype=np.float64))
+
+
+def build_system_device(data, grid, pbc=None, device=None, return_extra=False):
+    """``build_system`` with the tables built on the GPU (``mm_topology_build``): the same dict,
+    equal entry for entry (indices, masses summed in the same order, positions), for grids
+    where a host pass is slow. ``return_extra`` adds ``wrap_bits`` (the force kernels' image
+    shifts) and ``cell_point`` (linear grid index of every cell)."""
+    import ctypes
+
+    import torch
+
+    from . import _native
+
+    grid = np.asarray(grid)
+    g = np.ascontiguousarray(np.rint(grid).astype(np.int32))
+    if g.ndim != 3:
+        raise ValueError("grid must be three-dimensional")
+    pbc = _pbc_list(pbc)
+    ntypes = int(g.max()) if g.size else 0
+    type_mass = np.zeros(ntypes + 1)
+    for t in np.unique(g):
+        if t != 0:
+            type_mass[int(t)] = data[int(t)]["mass"]
+    spacing = np.ascontiguousarray(_majority_spacing(data, g), dtype=np.float64)
+    dev = torch.device(device if device is not None else "cuda")
+    dev_index = dev.index if dev.index is not None else torch.cuda.current_device()
+    lib = _native.lib()
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    handle = ctypes.c_void_p()
+    pbc_bits = sum(1 << d for d in range(3) if pbc[d])
+    rc = lib.mm_topology_build(ctypes.byref(handle), dev_index, vp(g), *[int(x) for x in g.shape],
+                               pbc_bits, vp(type_mass), ntypes, vp(spacing),
+                               _native.stream_ptr(torch.cuda.current_stream(dev_index)))
+    _native.check(rc, "mm_topology_build")
+    try:
+        C, N, B = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _native.check(lib.mm_topology_sizes(handle, ctypes.byref(C), ctypes.byref(N),
+                                            ctypes.byref(B)), "mm_topology_sizes")
+        C, N, B = C.value, N.value, B.value
+        sn = np.empty((C, 8), dtype=np.int64)
+        sc = np.empty((N, 8), dtype=np.int64)
+        bn = np.empty(B, dtype=np.int64)
+        masses = np.empty(N)
+        pos = np.empty((N, 3))
+        types = np.empty(C, dtype=np.int32)
+        wrap = np.empty(C, dtype=np.uint8)
+        cpt = np.empty(C, dtype=np.int32)
+        _native.check(lib.mm_topology_copy(handle, vp(sn), vp(sc), vp(bn), vp(masses), vp(pos),
+                                           vp(types), vp(wrap), vp(cpt),
+                                           _native.stream_ptr(torch.cuda.current_stream(dev_index))),
+                      "mm_topology_copy")
+    finally:
+        lib.mm_topology_destroy(handle)
+    dx, dy, dz = spacing
+    nx, ny, nz = g.shape
+    rvecs = []
+    if pbc[0]:
+        rvecs.append(np.array([nx * dx, 0.0, 0.0]))
+    if pbc[1]:
+        rvecs.append(np.array([0.0, ny * dy, 0.0]))
+    if pbc[2]:
+        rvecs.append(np.array([0.0, 0.0, nz * dz]))
+    rvecs = np.array(rvecs) if rvecs else np.zeros((0, 3))
+    output = {
+        "pos": pos,
+        "rvecs": rvecs,
+        "types": types.astype(np.int64),
+        "grid": grid,
+        "pbc": pbc,
+        "surrounding_cells": sc,
+        "surrounding_nodes": sn,
+        "boundary_nodes": bn,
+        "masses": masses,
+    }
+    for type_idx, type_data in data.items():
+        for key, value in type_data.items():
+            output["type%i/%s" % (type_idx, key)] = value
+    if return_extra:
+        output["wrap_bits"] = wrap
+        output["cell_point"] = cpt
     return output
 
 
