@@ -154,6 +154,7 @@ void free_field(mm_field* f) {
   (void)hipFree(f->kt);
   (void)hipFree(f->gcell);
   (void)hipFree(f->partials);
+  (void)hipFree(f->cgpart);
   (void)hipFree(f->nanflag);
   (void)hipFree(f->blk_sys);
   (void)hipFree(f->sys_cblk_ptr);
@@ -178,6 +179,26 @@ struct DeviceGuard {
     if (prev >= 0) (void)hipSetDevice(prev);
   }
 };
+
+// The topology, tables and scratch of a field object in a ForceArgs; positions, domain and
+// outputs are left to the caller.
+void field_args(const mm_field* f, ForceArgs& a) {
+  a.cnode = f->cnode;
+  a.cwrap = f->cwrap;
+  a.blk_row = f->blk_row;
+  a.tab = f->tab;
+  a.kt = f->kt;
+  a.nstates = f->nstates;
+  a.max_states = f->max_states;
+  a.W = f->W;
+  a.N = f->N;
+  a.ncell = f->ncell;
+  a.gcell = f->gcell;
+  a.partials = f->partials;
+  a.model = f->model;
+  a.nanflag = f->nanflag;
+  schedule_args(f, a);
+}
 
 }  // namespace
 
@@ -307,6 +328,7 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
       (rc = dmalloc(&f->nstates, nrows)) || (rc = dmalloc(&f->tab, tab.size())) ||
       (rc = dmalloc(&f->kt, kt.size())) || (rc = dmalloc(&f->gcell, (size_t)24 * W)) ||
       (rc = dmalloc(&f->partials, (size_t)kPartialStride * (nb > 0 ? nb : 1))) ||
+      (rc = dmalloc(&f->cgpart, (size_t)5 * (nnodes > 0 ? (nnodes + 255) / 256 : 1))) ||
       (rc = dmalloc(&f->nanflag, 1)) || (rc = dmalloc(&f->blk_sys, nb)) ||
       (rc = dmalloc(&f->sys_cblk_ptr, (size_t)nsys + 1)) ||
       (rc = dmalloc(&f->sys_cblk_idx, nb))) {
@@ -399,25 +421,11 @@ int mm_batch_compute(mm_field* f, const double* pos, const double* rvecs, double
   a.rvecs = rvecs;
   a.rvecs_stride = 9;
   a.blk_sys = f->blk_sys;
-  a.cnode = f->cnode;
-  a.cwrap = f->cwrap;
-  a.blk_row = f->blk_row;
-  a.tab = f->tab;
-  a.kt = f->kt;
-  a.nstates = f->nstates;
-  a.max_states = f->max_states;
-  a.W = f->W;
-  a.N = f->N;
-  a.ncell = f->ncell;
-  a.gcell = f->gcell;
-  a.partials = f->partials;
+  field_args(f, a);
   a.energy = nullptr;
   a.gpos = gpos;
   a.vtens = vtens;
   a.accumulate = 0;
-  a.model = f->model;
-  a.nanflag = f->nanflag;
-  schedule_args(f, a);
   a.force_grad = 1;
   hipError_t e = launch_cells(a, (hipStream_t)stream);
   if (e == hipSuccess && gpos) e = launch_gather(a, (hipStream_t)stream);
@@ -439,25 +447,11 @@ int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double
   ForceArgs a;
   a.pos = pos;
   a.rvecs = rvecs;
-  a.cnode = f->cnode;
-  a.cwrap = f->cwrap;
-  a.blk_row = f->blk_row;
-  a.tab = f->tab;
-  a.kt = f->kt;
-  a.nstates = f->nstates;
-  a.max_states = f->max_states;
-  a.W = f->W;
-  a.N = f->N;
-  a.ncell = f->ncell;
-  a.gcell = f->gcell;
-  a.partials = f->partials;
+  field_args(f, a);
   a.energy = energy;
   a.gpos = gpos;
   a.vtens = vtens;
   a.accumulate = accumulate;
-  a.model = f->model;
-  a.nanflag = f->nanflag;
-  schedule_args(f, a);
   hipError_t e = launch_force(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "force kernels");
   return MM_OK;
@@ -489,28 +483,67 @@ int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, v
   ForceArgs a;
   a.pos = pos;
   a.rvecs = rvecs;
-  a.cnode = f->cnode;
-  a.cwrap = f->cwrap;
-  a.blk_row = f->blk_row;
-  a.tab = f->tab;
-  a.kt = f->kt;
-  a.nstates = f->nstates;
-  a.max_states = f->max_states;
-  a.W = f->W;
-  a.N = f->N;
-  a.ncell = f->ncell;
-  a.gcell = f->gcell;
-  a.partials = f->partials;
+  field_args(f, a);
   a.energy = nullptr;
   a.gpos = nullptr;
   a.vtens = nullptr;
   a.accumulate = 0;
-  a.model = f->model;
-  a.nanflag = f->nanflag;
-  schedule_args(f, a);
   a.force_grad = 1;
   hipError_t e = launch_cells(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "cell kernel");
+  return MM_OK;
+}
+
+int mm_cg_trial(mm_field* f, const double* x, const double* g, double* d, double beta,
+                int set_dir, double alpha, const double* rvecs, double* xa, double* ga,
+                double* out, void* stream) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if (f->nsys != 1) return set_error(MM_ERR_ARG, "mm_cg_trial needs a single-system object");
+  if (!rvecs || !out) return set_error(MM_ERR_ARG, "rvecs and out are required");
+  if (f->N > 0 && (!x || !d || !xa || !ga || (set_dir && !g)))
+    return set_error(MM_ERR_ARG, "x, d, xa and ga (and g with set_dir) are required");
+  if (f->N > 0 && (xa == x || xa == d || ga == g || ga == d || xa == ga))
+    return set_error(MM_ERR_ARG, "xa and ga must not alias x, g, d or each other");
+  DeviceGuard guard(f->device);
+  hipStream_t s = (hipStream_t)stream;
+  ForceArgs a;
+  a.pos = xa;
+  a.rvecs = rvecs;
+  field_args(f, a);
+  a.energy = out;
+  a.gpos = ga;
+  a.vtens = nullptr;
+  a.accumulate = 0;
+  hipError_t e = launch_cg_trial(x, g, d, beta, set_dir, alpha, xa, 3 * f->N, s);
+  if (e == hipSuccess) e = launch_cells(a, s);
+  if (e == hipSuccess) e = launch_gather_dot(a, d, f->cgpart, s);
+  if (e == hipSuccess) {
+    ReduceList list;
+    add_force_totals(list, a, (int)(f->W / kCellBlock));
+    list.add(f->cgpart, f->N > 0 ? (f->N + 255) / 256 : 0, kReduceSum, out + 1, 0, MM_NAN_GPOS);
+    e = launch_reduce(list, f->nanflag, s);
+  }
+  if (e != hipSuccess) return hip_error(e, "line-search trial kernels");
+  return MM_OK;
+}
+
+int mm_cg_stats(mm_field* f, const double* x, const double* x_old, const double* g,
+                const double* g_old, double* out, void* stream) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if (!out) return set_error(MM_ERR_ARG, "out is required");
+  if (f->N > 0 && (!x || !x_old || !g || !g_old))
+    return set_error(MM_ERR_ARG, "x, x_old, g and g_old are required");
+  DeviceGuard guard(f->device);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (f->N > 0) {
+    e = launch_cg_stats(x, x_old, g, g_old, f->N, f->cgpart, out, f->nanflag, s);
+  } else {
+    ReduceList list;  // empty sums: five zeros
+    for (int q = 0; q < 5; ++q) list.add(f->cgpart, 0, kReduceSum, out + q, 0, 0);
+    e = launch_reduce(list, f->nanflag, s);
+  }
+  if (e != hipSuccess) return hip_error(e, "line-search statistics kernels");
   return MM_OK;
 }
 

@@ -172,48 +172,83 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   block_reduce_store<10, kCellBlock>(acc, red, partials + bid, nblocks);
 }
 
-// gpos[n] (+)= sum_{k=0..7} gcell[k][:][w(n,k)] in fixed k order (mmff.py:303-318).
+// gpos[n] (+)= sum_{k=0..7} gcell[k][:][w(n,k)] in fixed k order (mmff.py:303-318). With DOT,
+// also the block's partial of sum_n gpos[n] . dir[n] into dpart[blockIdx.x] (fixed order).
+template <bool DOT>
 __global__ void __launch_bounds__(256)
 gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell, int64_t N,
-              int64_t W, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag) {
+              int64_t W, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag,
+              const double* __restrict__ dir, double* __restrict__ dpart) {
   const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (n >= N) return;
-  double g[3] = {0.0, 0.0, 0.0};
+  if (!DOT && n >= N) return;
+  double dot[1] = {0.0};
+  if (n < N) {
+    double g[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int32_t e = ncell[k * N + n];
-    if (e >= 0) {
-      const int64_t w = e >> 3;
-      const int kk = e & 7;
+    for (int k = 0; k < 8; ++k) {
+      const int32_t e = ncell[k * N + n];
+      if (e >= 0) {
+        const int64_t w = e >> 3;
+        const int kk = e & 7;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
-    }
-  }
-  bool bad = false;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    bad |= (g[j] != g[j]);
-    gpos[3 * n + j] = accumulate ? gpos[3 * n + j] + g[j] : g[j];
-  }
-  if (bad) atomicOr(nanflag, 2);
-}
-
-__global__ void __launch_bounds__(1024)
-finalize_kernel(const double* __restrict__ partials, int nblocks, double* __restrict__ energy,
-                double* __restrict__ vtens, int accumulate, int* __restrict__ nanflag) {
-  double acc[10];
-  reduce_soa<10>(partials, nblocks, acc);
-  if (threadIdx.x == 0) {
-    int bad = 0;
-    if (acc[0] != acc[0]) bad |= 1;
-    if (energy) energy[0] = accumulate ? energy[0] + acc[0] : acc[0];
-    if (vtens) {
-      for (int q = 0; q < 9; ++q) {
-        if (acc[1 + q] != acc[1 + q]) bad |= 4;
-        vtens[q] = accumulate ? vtens[q] + acc[1 + q] : acc[1 + q];
+        for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
       }
     }
-    if (bad) atomicOr(nanflag, bad);
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      bad |= (g[j] != g[j]);
+      const double v = accumulate ? gpos[3 * n + j] + g[j] : g[j];
+      gpos[3 * n + j] = v;
+      if (DOT) dot[0] += v * dir[3 * n + j];
+    }
+    if (bad) atomicOr(nanflag, 2);
+  }
+  if (DOT) {
+    __shared__ double red[4];
+    block_reduce_store<1, 256>(dot, red, dpart + blockIdx.x, 1);
+  }
+}
+
+// NaN-propagating maximum (the result is NaN when either operand is).
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (a > b || a != a) ? a : b;
+}
+
+// One block per item: item i reduces src[0..n) (sum in a fixed order, or a NaN-propagating
+// maximum) into dst[0] (added to it when acc). A NaN total sets the item's bit in nanflag.
+__global__ void __launch_bounds__(1024)
+reduce_items_kernel(ReduceList list, int* __restrict__ nanflag) {
+  constexpr int kUnroll = 8;
+  __shared__ double red[16];
+  const ReduceItem it = list.item[blockIdx.x];
+  const bool is_max = it.op == kReduceMax;
+  const double zero = is_max ? -HUGE_VAL : 0.0;
+  double acc = zero;
+  for (int64_t base = threadIdx.x; base < it.n; base += 1024 * kUnroll) {
+    double v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t i = base + (int64_t)u * 1024;
+      v[u] = i < it.n ? it.src[i] : zero;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) acc = is_max ? nan_max(acc, v[u]) : acc + v[u];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(acc, off, 64);
+    acc = is_max ? nan_max(acc, o) : acc + o;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int w = 1; w < 16; ++w) t = is_max ? nan_max(t, red[w]) : t + red[w];
+    if (it.acc) t += it.dst[0];
+    it.dst[0] = t;
+    if (t != t && it.nanbit) atomicOr(nanflag, it.nanbit);
   }
 }
 
@@ -263,9 +298,32 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
 hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
-  hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N, a.W,
-                     a.gpos, a.accumulate, a.nanflag);
+  hipLaunchKernelGGL(gather_kernel<false>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
+                     a.W, a.gpos, a.accumulate, a.nanflag, nullptr, nullptr);
   return hipGetLastError();
+}
+
+hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, double* dpart,
+                             hipStream_t stream) {
+  if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
+  const int gb = (int)((a.N + 255) / 256);
+  hipLaunchKernelGGL(gather_kernel<true>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
+                     a.W, a.gpos, a.accumulate, a.nanflag, dir, dpart);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const ReduceList& list, int* nanflag, hipStream_t stream) {
+  if (list.count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_items_kernel, dim3(list.count), dim3(1024), 0, stream, list,
+                     nanflag);
+  return hipGetLastError();
+}
+
+void add_force_totals(ReduceList& list, const ForceArgs& a, int nb) {
+  if (a.energy) list.add(a.partials, nb, kReduceSum, a.energy, a.accumulate, 1);
+  if (a.vtens)
+    for (int q = 0; q < 9; ++q)
+      list.add(a.partials + (int64_t)(1 + q) * nb, nb, kReduceSum, a.vtens + q, a.accumulate, 4);
 }
 
 hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
@@ -274,9 +332,90 @@ hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
   if (err != hipSuccess) return err;
   err = launch_gather(a, stream);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, stream, a.partials, nb, a.energy,
-                     a.vtens, a.accumulate, a.nanflag);
+  ReduceList list;
+  add_force_totals(list, a, nb);
+  return launch_reduce(list, a.nanflag, stream);
+}
+
+// Trial point of a line search: optionally d <- beta d - g first, then xa <- x + alpha d.
+__global__ void __launch_bounds__(256)
+cg_trial_kernel(const double* __restrict__ x, const double* __restrict__ g,
+                double* __restrict__ d, double beta, int set_dir, double alpha,
+                double* __restrict__ xa, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double di;
+  if (set_dir) {
+    di = beta != 0.0 ? beta * d[i] - g[i] : -g[i];  // beta = 0: d is not read
+    d[i] = di;
+  } else {
+    di = d[i];
+  }
+  xa[i] = x[i] + alpha * di;
+}
+
+hipError_t launch_cg_trial(const double* x, const double* g, double* d, double beta, int set_dir,
+                           double alpha, double* xa, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cg_trial_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                     x, g, d, beta, set_dir, alpha, xa, n);
   return hipGetLastError();
+}
+
+// Per-block partials over the nodes n of |g_n|^2, g_n . go_n, |x_n - xo_n|^2 (sums, rows 0-2)
+// and of |g_n|^2, |x_n - xo_n|^2 (maxima, rows 3-4); part is [5][nb].
+__global__ void __launch_bounds__(256)
+cg_stats_kernel(const double* __restrict__ x, const double* __restrict__ xo,
+                const double* __restrict__ g, const double* __restrict__ go, int64_t N,
+                double* __restrict__ part, int nb) {
+  __shared__ double red[4 * 5];
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (n < N) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double gj = g[3 * n + j], dj = x[3 * n + j] - xo[3 * n + j];
+      v[0] += gj * gj;
+      v[1] += gj * go[3 * n + j];
+      v[2] += dj * dj;
+    }
+    v[3] = v[0];
+    v[4] = v[2];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const double o = __shfl_down(v[q], off, 64);
+      v[q] = q < 3 ? v[q] + o : nan_max(v[q], o);
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) red[wave * 5 + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int q = threadIdx.x;
+    double t = red[q];
+    for (int w = 1; w < 4; ++w) t = q < 3 ? t + red[w * 5 + q] : nan_max(t, red[w * 5 + q]);
+    part[(int64_t)q * nb + blockIdx.x] = t;
+  }
+}
+
+hipError_t launch_cg_stats(const double* x, const double* xo, const double* g, const double* go,
+                           int64_t N, double* part, double* out, int* nanflag,
+                           hipStream_t stream) {
+  if (N <= 0) return hipSuccess;
+  const int nb = (int)((N + 255) / 256);
+  hipLaunchKernelGGL(cg_stats_kernel, dim3(nb), dim3(256), 0, stream, x, xo, g, go, N, part, nb);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ReduceList list;
+  for (int q = 0; q < 5; ++q)
+    list.add(part + (int64_t)q * nb, nb, q < 3 ? kReduceSum : kReduceMax, out + q, 0, 0);
+  return launch_reduce(list, nanflag, stream);
 }
 
 // Per-system totals of a batch: one block per system sums its own cell blocks in list order.

@@ -39,7 +39,42 @@ struct ForceArgs {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
+// Reductions of device arrays into single values, one workgroup per item (reduce_items_kernel).
+constexpr int kReduceSum = 0;
+constexpr int kReduceMax = 1;
+constexpr int kReduceItems = 16;
+struct ReduceItem {
+  const double* src;
+  int64_t n;
+  double* dst;
+  int op;      // kReduceSum (fixed order) or kReduceMax (NaN-propagating)
+  int acc;     // add the total to dst[0] instead of overwriting it
+  int nanbit;  // bit set in the NaN flag when the total is NaN (0: none)
+};
+struct ReduceList {
+  ReduceItem item[kReduceItems];
+  int count = 0;
+  void add(const double* src, int64_t n, int op, double* dst, int acc, int nanbit) {
+    item[count++] = ReduceItem{src, n, dst, op, acc, nanbit};
+  }
+};
+hipError_t launch_reduce(const ReduceList& list, int* nanflag, hipStream_t stream);
+
 hipError_t launch_force(const ForceArgs& a, hipStream_t stream);
+// Energy (partial row 0) and virial (rows 1-9) items of a single-system cell pass with nb blocks.
+void add_force_totals(ReduceList& list, const ForceArgs& a, int nb);
+// Node gather that also writes the per-workgroup partials of sum_n gpos[n] . dir[n] into
+// dpart[(N + 255) / 256].
+hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, double* dpart,
+                             hipStream_t stream);
+// Line-search trial point over n values: d <- beta d - g when set_dir, then xa <- x + alpha d.
+hipError_t launch_cg_trial(const double* x, const double* g, double* d, double beta, int set_dir,
+                           double alpha, double* xa, int64_t n, hipStream_t stream);
+// out[0..5) = sum_n |g_n|^2, sum_n g_n . go_n, sum_n |x_n - xo_n|^2, max_n |g_n|^2,
+// max_n |x_n - xo_n|^2 over N nodes of 3 values; part holds 5 (N + 255) / 256 doubles.
+hipError_t launch_cg_stats(const double* x, const double* xo, const double* g, const double* go,
+                           int64_t N, double* part, double* out, int* nanflag,
+                           hipStream_t stream);
 // Per-system energy and virial of a batch: energies[s], vtens[9 s] (either may be null).
 hipError_t launch_batch_finalize(const double* partials, int nb, const int32_t* sys_ptr,
                                  const int32_t* sys_idx, int nsys, double* energies,

@@ -57,6 +57,115 @@ class _DeviceEnergy(object):
         return e, self.gpos, vt
 
 
+class _DeviceCGEngine(object):
+    """Device vectors of a conjugate-gradient search over all node positions of a
+    ``CartesianDOF`` (the engine protocol of ``sampling.opt.VectorEngine``).
+
+    A trial point is one ``mm_cg_trial`` sequence: the pending direction update
+    d <- beta d - g and the point x + a d in one kernel, the force kernels, the slope g.d reduced
+    inside the node gather, then one read of two doubles. An accepted step is one
+    ``mm_cg_stats`` sequence whose sums and maxima give both the Polak-Ribiere coefficient and
+    the gpos/dpos measures of ``check_convergence`` (dpos against the previous accepted point,
+    which is where the last check left ``_last_pos``). Trial points alternate between two
+    buffers, so no vector is copied.
+    """
+
+    def __init__(self, dof, x0):
+        self.energy = dof._device
+        self.field = self.energy.fpm.field
+        kw = dict(dtype=torch.float64, device=x0.device)
+        n = x0.numel()
+        self.x = x0.detach().reshape(-1).clone()
+        self.g = torch.zeros(n, **kw)
+        self.d = torch.zeros(n, **kw)
+        self._slots = [[torch.empty(n, **kw), torch.empty(n, **kw)] for _ in range(2)]
+        self._out = torch.zeros(8, **kw)
+        self._last = None
+        self._best = None
+        self._beta = 0.0
+        self._set_dir = True  # d <- beta d - g happens in the next trial
+        self._dp = 0.0
+        self.gg = None
+        self._measures = None
+        self._checked = False
+
+    def _read(self, k):
+        return self._out[:k].tolist()
+
+    def _energy(self, e):
+        if np.isnan(e):
+            raise ValueError("The energy is not-a-number (``nan``).")
+        press = self.energy.press
+        if press is not None:
+            e += press.pext * self.energy.fpm.system.domain.volume
+        return e
+
+    def start(self):
+        en = self.energy
+        en.calls += 1
+        en.fpm.compute_device(self.x.view(-1, 3), en.rvecs, self._out[0:1], self.g.view(-1, 3))
+        self.field.cg_stats(self.x, self.x, self.g, self.g, self._out[1:6])
+        e, self.gg = self._read(2)
+        self._beta, self._set_dir, self._dp = 0.0, True, 0.0
+        self._measures, self._checked, self._best = None, False, None
+        return self._energy(e), self.gg
+
+    def slope(self):
+        if self._set_dir:
+            # g.(beta d - g) with g.d = dp, the slope of the accepted trial
+            return -self.gg + self._beta * self._dp
+        return float(torch.dot(self.g, self.d).item())
+
+    def restart(self):
+        self._beta, self._set_dir, self._dp = 0.0, True, 0.0
+
+    def trial(self, a):
+        k = 1 if self._best == 0 else 0
+        xa, ga = self._slots[k]
+        en = self.energy
+        en.calls += 1
+        self.field.cg_trial(self.x, self.g, self.d, self._beta, self._set_dir, a, en.rvecs, xa,
+                            ga, self._out)
+        self._set_dir = False
+        e, dp = self._read(2)
+        self._last = k
+        return self._energy(e), dp
+
+    def keep(self):
+        self._best = self._last
+
+    def accept(self):
+        k = self._best
+        xa, ga = self._slots[k]
+        self.field.cg_stats(xa, self.x, ga, self.g, self._out)
+        gsq, g_gold, dsq, gsq_max, dsq_max = self._read(5)
+        self._slots[k] = [self.x, self.g]
+        self.x, self.g = xa, ga
+        self._best = None
+        self._measures = (gsq, gsq_max, dsq, dsq_max)
+        self.gg = gsq
+        return gsq, gsq - g_gold
+
+    def set_direction(self, beta, dp):
+        self._beta, self._set_dir, self._dp = beta, True, dp
+
+    def take_measures(self):
+        """(sum, max of |g_n|^2, sum, max of |x_n - x_last_n|^2) for a convergence check, or
+        None for the first check after ``start``."""
+        if not self._checked:
+            self._checked = True
+            self._measures = None
+            return None
+        m = self._measures
+        if m is None:
+            # no step since the last check: the displacement is zero
+            self.field.cg_stats(self.x, self.x, self.g, self.g, self._out)
+            gsq, _, _, gsq_max, _ = self._read(5)
+            m = (gsq, gsq_max, 0.0, 0.0)
+        self._measures = None
+        return m
+
+
 class DOF(object):
     def __init__(self, mmf):
         self.mmf = mmf
@@ -104,6 +213,7 @@ class CartesianDOF(DOF):
         DOF.__init__(self, mmf)
         self._last_pos = None
         self._last_pos_dev = None
+        self._cg_engine = None
         try:
             self._device = _DeviceEnergy(mmf)
         except NotImplementedError:
@@ -129,8 +239,20 @@ class CartesianDOF(DOF):
             self._pos[self.select] = x.reshape(-1, 3)
         self.mmf.update_pos(self._pos[:])
 
+    def cg_engine(self, x0):
+        """Fused device vectors for ``ConjugateGradientSearch`` starting at the device tensor
+        x0 (all nodes free, device force field), or None. While the engine drives the search,
+        ``check_convergence`` takes its measures from the engine's accepted steps."""
+        if self._device is None or self.select is not None or \
+                not (isinstance(x0, torch.Tensor) and x0.is_cuda):
+            return None
+        self._cg_engine = _DeviceCGEngine(self, x0)
+        return self._cg_engine
+
     def fun(self, x, do_gradient=False):
         """Energy and, with do_gradient, its gradient with respect to x (dof.py:132-153)."""
+        # an evaluation outside the engine: the engine no longer holds the current state
+        self._cg_engine = None
         if isinstance(x, torch.Tensor) and x.is_cuda:
             if self._device is None:
                 raise NotImplementedError("device evaluation needs one ForcePartMechanical")
@@ -197,6 +319,22 @@ class CartesianDOF(DOF):
     def check_convergence(self):
         """Convergence criteria of dof.py:155-193. On the device path the measures are reduced
         on the GPU; the host arrays are refreshed by ``sync_from_device`` when needed."""
+        if self._cg_engine is not None:
+            m = self._cg_engine.take_measures()
+            if m is None:
+                self.converged = False
+                self.conv_val = 2
+                self.conv_worst = "first_step"
+                self.conv_count = -1
+                return
+            gsq_sum, gsq_max, dsq_sum, dsq_max = m
+            nn = self._cg_engine.x.numel() // 3
+            self.gpos_max, self.gpos_rms = np.sqrt(gsq_max), np.sqrt(gsq_sum / nn)
+            self.dpos_max, self.dpos_rms = np.sqrt(dsq_max), np.sqrt(dsq_sum / nn)
+            self._finish_convergence()
+            if self.converged:
+                self.sync_from_device()
+            return
         if getattr(self, "_pos_dev", None) is not None and getattr(self, "_gpos_dev", None) \
                 is not None:
             first, (self.gpos_max, self.gpos_rms, self.dpos_max, self.dpos_rms) = \

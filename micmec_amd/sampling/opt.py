@@ -18,7 +18,8 @@ from ..log import log
 from .iterative import (AttributeStateItem, DomainStateItem, EPotContribStateItem, Hook,
                         Iterative, PosStateItem, VolumeStateItem)
 
-__all__ = ["OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch"]
+__all__ = ["OptScreenLog", "BaseOptimizer", "CGOptimizer", "ConjugateGradientSearch",
+           "VectorEngine"]
 
 
 class OptScreenLog(Hook):
@@ -91,33 +92,89 @@ def _dot(a, b):
     return float(np.dot(a, b))
 
 
+class VectorEngine(object):
+    """The vectors of a conjugate-gradient search (position x, gradient g, direction d) as numpy
+    arrays or torch tensors, each energy and gradient one call of ``fun(x, True)``.
+
+    The search talks to its vectors only through these methods, so a device implementation
+    can fuse them (see ``CartesianDOF.cg_engine``):
+
+    ``start()`` evaluates f and g at x, sets d = -g and returns (f, g.g); ``slope()`` is g.d;
+    ``restart()`` sets d = -g; ``trial(a)`` evaluates the point x + a d and returns (f, g'.d)
+    there; ``keep()`` marks the last trial as the best of the line search; ``accept()`` moves
+    to the kept point and returns (g.g, g.(g - g_old)) there; ``set_direction(beta, dp)`` sets
+    d = -g + beta d, where dp is g.d with the old d.
+    """
+
+    def __init__(self, x0, fun):
+        self.x = x0
+        self.fun = fun
+        self.f = None
+        self.g = None
+        self.d = None
+
+    def start(self):
+        self.f, self.g = self.fun(self.x, True)
+        self.d = -self.g
+        return self.f, _dot(self.g, self.g)
+
+    def slope(self):
+        return _dot(self.g, self.d)
+
+    def restart(self):
+        self.d = -self.g
+
+    def trial(self, a):
+        xa = self.x + a * self.d
+        fa, ga = self.fun(xa, True)
+        self._last = (xa, fa, ga)
+        return fa, _dot(ga, self.d)
+
+    def keep(self):
+        self._best = self._last
+
+    def accept(self):
+        xa, fa, ga = self._best
+        g_old = self.g
+        self.x, self.f, self.g = xa, fa, ga
+        return _dot(ga, ga), _dot(ga, ga - g_old)
+
+    def set_direction(self, beta, dp):
+        self.d = -self.g + beta * self.d
+
+
 class ConjugateGradientSearch(object):
     """Polak-Ribiere (PR+) conjugate gradient with a secant line search on phi'(alpha).
 
     One ``propagate`` is one line search and one direction update, the unit of one iteration
-    of the reference's minimiser.
+    of the reference's minimiser. The vectors live in ``engine`` (a ``VectorEngine`` over
+    ``fun`` unless another one is given).
     """
 
-    def __init__(self, x0, fun, max_ls=20, curvature=1e-3, max_step=None):
-        self.x = x0
+    def __init__(self, x0, fun, max_ls=20, curvature=1e-3, max_step=None, engine=None):
+        self.engine = engine if engine is not None else VectorEngine(x0, fun)
         self.fun = fun
         self.max_ls = max_ls
         self.curvature = curvature
         self.max_step = max_step
 
+    x = property(lambda self: self.engine.x)
+    g = property(lambda self: self.engine.g)
+    d = property(lambda self: self.engine.d)
+
     def initialize(self):
-        self.f, self.g = self.fun(self.x, True)
-        self.d = -self.g
-        gnorm = np.sqrt(_dot(self.g, self.g))
+        self.f, self.gg = self.engine.start()
+        gnorm = np.sqrt(self.gg)
         self.alpha = 1.0 / gnorm if gnorm > 0 else 1.0
         self.restarts = 0
 
     def propagate(self):
-        dphi0 = _dot(self.g, self.d)
+        E = self.engine
+        dphi0 = E.slope()
         if not dphi0 < 0.0:
             # not a descent direction: restart along the steepest descent
-            self.d = -self.g
-            dphi0 = _dot(self.g, self.d)
+            E.restart()
+            dphi0 = E.slope()
             self.restarts += 1
             if dphi0 == 0.0:
                 return True
@@ -131,12 +188,11 @@ class ConjugateGradientSearch(object):
         a = self.alpha
         best = None
         for _ in range(self.max_ls):
-            xa = self.x + a * self.d
-            fa, ga = self.fun(xa, True)
-            dp = _dot(ga, self.d)
+            fa, dp = E.trial(a)
             ok = np.isfinite(fa) and np.isfinite(dp) and fa <= f0 + ftol
-            if ok and (best is None or abs(dp) < abs(best[4])):
-                best = (a, fa, ga, xa, dp)
+            if ok and (best is None or abs(dp) < abs(best[2])):
+                best = (a, fa, dp)
+                E.keep()
             if ok and abs(dp) <= self.curvature * abs(dphi0):
                 break
             if ok and dp < 0.0:
@@ -164,13 +220,12 @@ class ConjugateGradientSearch(object):
             a = a_new
         if best is None:
             return False
-        a, fa, ga, xa, _ = best
-        g_old = self.g
-        self.x, self.f, self.g = xa, fa, ga
-        gg = _dot(g_old, g_old)
-        beta = max(0.0, _dot(ga, ga - g_old) / gg) if gg > 0 else 0.0
-        self.d = -ga + beta * self.d
-        self.alpha = a
+        a, fa, dp = best
+        gg_old = self.gg
+        gg, num = E.accept()
+        beta = max(0.0, num / gg_old) if gg_old > 0 else 0.0
+        E.set_direction(beta, dp)
+        self.f, self.gg, self.alpha = fa, gg, a
         return True
 
 
@@ -180,14 +235,20 @@ class CGOptimizer(BaseOptimizer):
 
     def __init__(self, dof, state=None, hooks=None, counter0=0, device=True):
         x0 = dof.x0
+        engine = None
         if device and getattr(dof, "device", None) is not None:
             x0 = torch.as_tensor(np.ascontiguousarray(x0), dtype=torch.float64,
                                  device=dof.device)
-        self.minimizer = ConjugateGradientSearch(x0, self.fun)
+            make = getattr(dof, "cg_engine", None)
+            engine = make(x0) if make is not None else None
+        self.minimizer = ConjugateGradientSearch(x0, self.fun, engine=engine)
         BaseOptimizer.__init__(self, dof, state, hooks, counter0)
 
     def initialize(self):
         self.minimizer.initialize()
+        self.epot = self.minimizer.f
+        self.x = self.minimizer.x
+        self.dof._accept(self.x, self.minimizer.g)
         BaseOptimizer.initialize(self)
 
     def propagate(self):
