@@ -2,8 +2,8 @@
  *
  * Plain pointers and sizes only. Device pointers are HIP device memory of the device the
  * object was created on; streams are hipStream_t passed as void* (NULL = default stream).
- * All compute calls are asynchronous on the given stream; only mm_field_check and the
- * *_read functions synchronise.
+ * All compute calls are asynchronous on the given stream; only mm_field_check, the *_read
+ * functions and the mm_cg_* calls given a host output synchronise.
  *
  * Status codes: 0 = ok; MM_ERR_ARG -> ValueError/TypeError in the Python layer;
  * MM_ERR_NAN -> ValueError (reference: micmec/pes/mmff.py:135-148); MM_ERR_HIP and
@@ -98,24 +98,29 @@ int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, v
 /* One trial point of a conjugate-gradient line search over the node positions (single-system
  * objects). Replaces the work of one CartesianDOF.fun call (micmec/sampling/dof.py:132-153)
  * together with the vector arithmetic around it in the minimiser that CGOptimizer builds
- * (micmec/sampling/opt.py:153-162, molmod.minimizer's ConjugateGradient and NewtonLineSearch):
+ * (micmec/sampling/opt.py:153-162, molmod.minimizer's ConjugateGradient and NewtonLineSearch)
+ * and the measures of CartesianDOF.check_convergence (dof.py:155-193) should the point be
+ * accepted:
  *   if set_dir != 0: d <- beta d - g   (the new search direction; beta = 0 restarts along -g);
  *   xa <- x + alpha d;  energy and gradient ga at xa and the domain rvecs;
- *   out[0] = energy, out[1] = sum_n ga[n] . d[n]   (the slope of the line at alpha).
- * x, g, d, xa, ga are [nnodes][3] and out [2], all device pointers; xa and ga must not alias
- * x, g, d or each other. Asynchronous on `stream`; calls on one object share its scratch, so
- * issue them in order on one stream. */
+ *   out[0] = energy, out[1] = sum_n ga_n . d_n (the slope of the line at alpha),
+ *   out[2] = sum_n |ga_n|^2, out[3] = sum_n ga_n . g_n (Polak-Ribiere numerator out[2] - out[3]),
+ *   out[4] = max_n |ga_n|^2, out[5] = sum_n |xa_n - x_n|^2, out[6] = max_n |xa_n - x_n|^2.
+ * x, g, d, xa, ga are [nnodes][3] and out [7], all device pointers; xa and ga must not alias
+ * x, g, d or each other. Asynchronous on `stream`, unless host_out (host, [7]) is given: then
+ * the call waits for the results and copies them there. Calls on one object share its scratch,
+ * so issue them in order on one stream. */
 int mm_cg_trial(mm_field* f, const double* x, const double* g, double* d, double beta,
                 int set_dir, double alpha, const double* rvecs, double* xa, double* ga,
-                double* out, void* stream);
+                double* out, double* host_out, void* stream);
 
-/* Statistics of an accepted line-search step: out[0] = sum_n |g_n|^2, out[1] = sum_n g_n . g_old_n
- * (the Polak-Ribiere numerator is out[0] - out[1]), out[2] = sum_n |x_n - x_old_n|^2,
- * out[3] = max_n |g_n|^2, out[4] = max_n |x_n - x_old_n|^2 (the gpos and dpos measures of
- * CartesianDOF.check_convergence, micmec/sampling/dof.py:155-193, before the square root and the
- * mean). Device pointers, x, x_old, g, g_old [nnodes][3], out [5]; asynchronous on `stream`. */
+/* The same measures at given points: out[0] = sum_n |g_n|^2, out[1] = sum_n g_n . g_old_n,
+ * out[2] = sum_n |x_n - x_old_n|^2, out[3] = max_n |g_n|^2, out[4] = max_n |x_n - x_old_n|^2
+ * (CartesianDOF.check_convergence, micmec/sampling/dof.py:165-181, before the square root and
+ * the mean). Device pointers, x, x_old, g, g_old [nnodes][3], out [5]; asynchronous on
+ * `stream` unless host_out (host, [5]) is given, as for mm_cg_trial. */
 int mm_cg_stats(mm_field* f, const double* x, const double* x_old, const double* g,
-                const double* g_old, double* out, void* stream);
+                const double* g_old, double* out, double* host_out, void* stream);
 
 /* Sizes of the object: cells, nodes, padded work slots. */
 int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* nslots);

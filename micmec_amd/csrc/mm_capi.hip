@@ -155,6 +155,7 @@ void free_field(mm_field* f) {
   (void)hipFree(f->gcell);
   (void)hipFree(f->partials);
   (void)hipFree(f->cgpart);
+  if (f->cghost) (void)hipHostFree(f->cghost);
   (void)hipFree(f->nanflag);
   (void)hipFree(f->blk_sys);
   (void)hipFree(f->sys_cblk_ptr);
@@ -328,7 +329,7 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
       (rc = dmalloc(&f->nstates, nrows)) || (rc = dmalloc(&f->tab, tab.size())) ||
       (rc = dmalloc(&f->kt, kt.size())) || (rc = dmalloc(&f->gcell, (size_t)24 * W)) ||
       (rc = dmalloc(&f->partials, (size_t)kPartialStride * (nb > 0 ? nb : 1))) ||
-      (rc = dmalloc(&f->cgpart, (size_t)5 * (nnodes > 0 ? (nnodes + 255) / 256 : 1))) ||
+      (rc = dmalloc(&f->cgpart, (size_t)8 * (nnodes > 0 ? (nnodes + 255) / 256 : 1))) ||
       (rc = dmalloc(&f->nanflag, 1)) || (rc = dmalloc(&f->blk_sys, nb)) ||
       (rc = dmalloc(&f->sys_cblk_ptr, (size_t)nsys + 1)) ||
       (rc = dmalloc(&f->sys_cblk_idx, nb))) {
@@ -347,6 +348,8 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
     return rc;
   }
   hipError_t e = hipMemset(f->nanflag, 0, sizeof(int));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&f->cghost, 8 * sizeof(double), hipHostMallocDefault);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming);
@@ -494,15 +497,25 @@ int mm_field_launch_cells(mm_field* f, const double* pos, const double* rvecs, v
   return MM_OK;
 }
 
+// Copy n results from the device to host_out through the object's pinned staging, after the
+// work queued on the stream (a synchronising read).
+static int read_results(mm_field* f, const double* out, double* host_out, int n, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(f->cghost, out, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_error(e, "reading line-search results");
+  std::memcpy(host_out, f->cghost, sizeof(double) * n);
+  return MM_OK;
+}
+
 int mm_cg_trial(mm_field* f, const double* x, const double* g, double* d, double beta,
                 int set_dir, double alpha, const double* rvecs, double* xa, double* ga,
-                double* out, void* stream) {
+                double* out, double* host_out, void* stream) {
   if (!f) return set_error(MM_ERR_ARG, "field is NULL");
   if (f->nsys != 1) return set_error(MM_ERR_ARG, "mm_cg_trial needs a single-system object");
   if (!rvecs || !out) return set_error(MM_ERR_ARG, "rvecs and out are required");
-  if (f->N > 0 && (!x || !d || !xa || !ga || (set_dir && !g)))
-    return set_error(MM_ERR_ARG, "x, d, xa and ga (and g with set_dir) are required");
-  if (f->N > 0 && (xa == x || xa == d || ga == g || ga == d || xa == ga))
+  if (f->N > 0 && (!x || !g || !d || !xa || !ga))
+    return set_error(MM_ERR_ARG, "x, g, d, xa and ga are required");
+  if (f->N > 0 && (xa == x || xa == g || xa == d || ga == x || ga == g || ga == d || xa == ga))
     return set_error(MM_ERR_ARG, "xa and ga must not alias x, g, d or each other");
   DeviceGuard guard(f->device);
   hipStream_t s = (hipStream_t)stream;
@@ -514,21 +527,28 @@ int mm_cg_trial(mm_field* f, const double* x, const double* g, double* d, double
   a.gpos = ga;
   a.vtens = nullptr;
   a.accumulate = 0;
-  hipError_t e = launch_cg_trial(x, g, d, beta, set_dir, alpha, xa, 3 * f->N, s);
+  const int64_t nbn = f->N > 0 ? (f->N + 255) / 256 : 0;
+  double* tpart = f->cgpart;            // rows 0-1: displacement sum and maximum
+  double* gpart = f->cgpart + 2 * nbn;  // rows 2-5: g.d, g.g, g.g_old, max |g_n|^2
+  hipError_t e = launch_cg_trial(x, g, d, beta, set_dir, alpha, xa, f->N, tpart, s);
   if (e == hipSuccess) e = launch_cells(a, s);
-  if (e == hipSuccess) e = launch_gather_dot(a, d, f->cgpart, s);
+  if (e == hipSuccess) e = launch_gather_dot(a, d, g, gpart, s);
   if (e == hipSuccess) {
     ReduceList list;
     add_force_totals(list, a, (int)(f->W / kCellBlock));
-    list.add(f->cgpart, f->N > 0 ? (f->N + 255) / 256 : 0, kReduceSum, out + 1, 0, MM_NAN_GPOS);
+    const int maxop = nbn > 0 ? kReduceMax : kReduceSum;  // an empty maximum is 0
+    for (int q = 0; q < 4; ++q)
+      list.add(gpart + q * nbn, nbn, q < 3 ? kReduceSum : maxop, out + 1 + q, 0, MM_NAN_GPOS);
+    list.add(tpart, nbn, kReduceSum, out + 5, 0, 0);
+    list.add(tpart + nbn, nbn, maxop, out + 6, 0, 0);
     e = launch_reduce(list, f->nanflag, s);
   }
   if (e != hipSuccess) return hip_error(e, "line-search trial kernels");
-  return MM_OK;
+  return host_out ? read_results(f, out, host_out, 7, s) : MM_OK;
 }
 
 int mm_cg_stats(mm_field* f, const double* x, const double* x_old, const double* g,
-                const double* g_old, double* out, void* stream) {
+                const double* g_old, double* out, double* host_out, void* stream) {
   if (!f) return set_error(MM_ERR_ARG, "field is NULL");
   if (!out) return set_error(MM_ERR_ARG, "out is required");
   if (f->N > 0 && (!x || !x_old || !g || !g_old))
@@ -544,7 +564,7 @@ int mm_cg_stats(mm_field* f, const double* x, const double* x_old, const double*
     e = launch_reduce(list, f->nanflag, s);
   }
   if (e != hipSuccess) return hip_error(e, "line-search statistics kernels");
-  return MM_OK;
+  return host_out ? read_results(f, out, host_out, 5, s) : MM_OK;
 }
 
 int mm_field_info(const mm_field* f, int64_t* ncells, int64_t* nnodes, int64_t* nslots) {

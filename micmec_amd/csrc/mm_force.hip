@@ -172,18 +172,71 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   block_reduce_store<10, kCellBlock>(acc, red, partials + bid, nblocks);
 }
 
-// gpos[n] (+)= sum_{k=0..7} gcell[k][:][w(n,k)] in fixed k order (mmff.py:303-318). With DOT,
-// also the block's partial of sum_n gpos[n] . dir[n] into dpart[blockIdx.x] (fixed order).
+// NaN-propagating maximum (the result is NaN when either operand is).
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (a > b || a != a) ? a : b;
+}
+
+// Block reduction of NS sums (fixed order) followed by NM NaN-propagating maxima; thread q
+// writes total q to part[q * stride] (256-thread blocks).
+template <int NS, int NM>
+__device__ __forceinline__ void block_sum_max_store(double (&v)[NS + NM], double* part,
+                                                    int64_t stride) {
+  constexpr int NV = NS + NM;
+  __shared__ double red[4 * NV];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double o = __shfl_down(v[q], off, 64);
+      v[q] = q < NS ? v[q] + o : nan_max(v[q], o);
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[wave * NV + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int q = threadIdx.x;
+    double t = red[q];
+    for (int w = 1; w < 4; ++w) t = q < NS ? t + red[w * NV + q] : nan_max(t, red[w * NV + q]);
+    part[q * stride] = t;
+  }
+}
+
+// gpos[n] (+)= sum_{k=0..7} gcell[k][:][w(n,k)] in fixed k order (mmff.py:303-318). A workgroup
+// owns 256 consecutive nodes; their 768 gpos values (and, with DOT, dir and gold values) move
+// between HBM and LDS in contiguous runs, and each thread takes its node's three from LDS. With
+// DOT (a line-search trial), also the workgroup partials of sum_n gpos_n . dir_n,
+// sum_n |gpos_n|^2, sum_n gpos_n . gold_n and max_n |gpos_n|^2 into dpart rows 0-3 (row stride =
+// workgroups).
 template <bool DOT>
 __global__ void __launch_bounds__(256)
 gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell, int64_t N,
               int64_t W, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag,
-              const double* __restrict__ dir, double* __restrict__ dpart) {
-  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (!DOT && n >= N) return;
-  double dot[1] = {0.0};
+              const double* __restrict__ dir, const double* __restrict__ gold,
+              double* __restrict__ dpart) {
+  __shared__ double sg[768];
+  __shared__ double sd[DOT ? 768 : 1], so[DOT ? 768 : 1];
+  const int t = threadIdx.x;
+  const int64_t n0 = (int64_t)blockIdx.x * 256, n = n0 + t;
+  const int64_t e0 = 3 * n0;
+  const int64_t ne = 3 * (N < n0 + 256 ? N - n0 : (int64_t)256);  // values of this workgroup
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int li = r * 256 + t;
+    if (li < ne) {
+      if (accumulate) sg[li] = gpos[e0 + li];
+      if (DOT) {
+        sd[li] = dir[e0 + li];
+        so[li] = gold[e0 + li];
+      }
+    }
+  }
+  double g[3] = {0.0, 0.0, 0.0};
   if (n < N) {
-    double g[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int32_t e = ncell[k * N + n];
@@ -194,25 +247,32 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
         for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
       }
     }
+  }
+  __syncthreads();
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (n < N) {
     bool bad = false;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       bad |= (g[j] != g[j]);
-      const double v = accumulate ? gpos[3 * n + j] + g[j] : g[j];
-      gpos[3 * n + j] = v;
-      if (DOT) dot[0] += v * dir[3 * n + j];
+      const double v = accumulate ? sg[3 * t + j] + g[j] : g[j];
+      sg[3 * t + j] = v;
+      if (DOT) {
+        acc[0] += v * sd[3 * t + j];
+        acc[1] += v * v;
+        acc[2] += v * so[3 * t + j];
+      }
     }
+    acc[3] = acc[1];
     if (bad) atomicOr(nanflag, 2);
   }
-  if (DOT) {
-    __shared__ double red[4];
-    block_reduce_store<1, 256>(dot, red, dpart + blockIdx.x, 1);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int li = r * 256 + t;
+    if (li < ne) gpos[e0 + li] = sg[li];
   }
-}
-
-// NaN-propagating maximum (the result is NaN when either operand is).
-__device__ __forceinline__ double nan_max(double a, double b) {
-  return (a > b || a != a) ? a : b;
+  if (DOT) block_sum_max_store<3, 1>(acc, dpart + blockIdx.x, gridDim.x);
 }
 
 // One block per item: item i reduces src[0..n) (sum in a fixed order, or a NaN-propagating
@@ -299,16 +359,16 @@ hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
   hipLaunchKernelGGL(gather_kernel<false>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                     a.W, a.gpos, a.accumulate, a.nanflag, nullptr, nullptr);
+                     a.W, a.gpos, a.accumulate, a.nanflag, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, double* dpart,
-                             hipStream_t stream) {
+hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, const double* gold,
+                             double* dpart, hipStream_t stream) {
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
   hipLaunchKernelGGL(gather_kernel<true>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                     a.W, a.gpos, a.accumulate, a.nanflag, dir, dpart);
+                     a.W, a.gpos, a.accumulate, a.nanflag, dir, gold, dpart);
   return hipGetLastError();
 }
 
@@ -337,28 +397,53 @@ hipError_t launch_force(const ForceArgs& a, hipStream_t stream) {
   return launch_reduce(list, a.nanflag, stream);
 }
 
-// Trial point of a line search: optionally d <- beta d - g first, then xa <- x + alpha d.
+// Trial point of a line search over 256 nodes a workgroup: optionally d <- beta d - g first,
+// then xa <- x + alpha d, in contiguous runs of the workgroup's 768 values; the squared
+// displacements go through LDS to the node threads for the workgroup partials of
+// sum_n |xa_n - x_n|^2 and max_n |xa_n - x_n|^2 (part rows 0-1, row stride = workgroups).
 __global__ void __launch_bounds__(256)
 cg_trial_kernel(const double* __restrict__ x, const double* __restrict__ g,
                 double* __restrict__ d, double beta, int set_dir, double alpha,
-                double* __restrict__ xa, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  double di;
-  if (set_dir) {
-    di = beta != 0.0 ? beta * d[i] - g[i] : -g[i];  // beta = 0: d is not read
-    d[i] = di;
-  } else {
-    di = d[i];
+                double* __restrict__ xa, int64_t N, double* __restrict__ part) {
+  __shared__ double dx2[768];
+  const int t = threadIdx.x;
+  const int64_t n0 = (int64_t)blockIdx.x * 256;
+  const int64_t e0 = 3 * n0;
+  const int64_t ne = 3 * (N < n0 + 256 ? N - n0 : (int64_t)256);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int li = r * 256 + t;
+    double v = 0.0;
+    if (li < ne) {
+      const int64_t i = e0 + li;
+      double di;
+      if (set_dir) {
+        di = beta != 0.0 ? beta * d[i] - g[i] : -g[i];  // beta = 0: d is not read
+        d[i] = di;
+      } else {
+        di = d[i];
+      }
+      const double xi = x[i];
+      const double xn = xi + alpha * di;
+      xa[i] = xn;
+      const double dx = xn - xi;
+      v = dx * dx;
+    }
+    dx2[li] = v;
   }
-  xa[i] = x[i] + alpha * di;
+  __syncthreads();
+  double acc[2];
+  acc[0] = (dx2[3 * t] + dx2[3 * t + 1]) + dx2[3 * t + 2];
+  acc[1] = acc[0];
+  block_sum_max_store<1, 1>(acc, part + blockIdx.x, gridDim.x);
 }
 
 hipError_t launch_cg_trial(const double* x, const double* g, double* d, double beta, int set_dir,
-                           double alpha, double* xa, int64_t n, hipStream_t stream) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(cg_trial_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
-                     x, g, d, beta, set_dir, alpha, xa, n);
+                           double alpha, double* xa, int64_t N, double* part,
+                           hipStream_t stream) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cg_trial_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
+                     x, g, d, beta, set_dir, alpha, xa, N, part);
   return hipGetLastError();
 }
 
@@ -367,8 +452,7 @@ hipError_t launch_cg_trial(const double* x, const double* g, double* d, double b
 __global__ void __launch_bounds__(256)
 cg_stats_kernel(const double* __restrict__ x, const double* __restrict__ xo,
                 const double* __restrict__ g, const double* __restrict__ go, int64_t N,
-                double* __restrict__ part, int nb) {
-  __shared__ double red[4 * 5];
+                double* __restrict__ part) {
   const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (n < N) {
@@ -382,26 +466,7 @@ cg_stats_kernel(const double* __restrict__ x, const double* __restrict__ xo,
     v[3] = v[0];
     v[4] = v[2];
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      const double o = __shfl_down(v[q], off, 64);
-      v[q] = q < 3 ? v[q] + o : nan_max(v[q], o);
-    }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) red[wave * 5 + q] = v[q];
-  }
-  __syncthreads();
-  if (threadIdx.x < 5) {
-    const int q = threadIdx.x;
-    double t = red[q];
-    for (int w = 1; w < 4; ++w) t = q < 3 ? t + red[w * 5 + q] : nan_max(t, red[w * 5 + q]);
-    part[(int64_t)q * nb + blockIdx.x] = t;
-  }
+  block_sum_max_store<3, 2>(v, part + blockIdx.x, gridDim.x);
 }
 
 hipError_t launch_cg_stats(const double* x, const double* xo, const double* g, const double* go,
@@ -409,7 +474,7 @@ hipError_t launch_cg_stats(const double* x, const double* xo, const double* g, c
                            hipStream_t stream) {
   if (N <= 0) return hipSuccess;
   const int nb = (int)((N + 255) / 256);
-  hipLaunchKernelGGL(cg_stats_kernel, dim3(nb), dim3(256), 0, stream, x, xo, g, go, N, part, nb);
+  hipLaunchKernelGGL(cg_stats_kernel, dim3(nb), dim3(256), 0, stream, x, xo, g, go, N, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   ReduceList list;

@@ -34,7 +34,8 @@ struct mm_field {
   double* kt = nullptr;
   double* gcell = nullptr;
   double* partials = nullptr;
-  double* cgpart = nullptr;           // [5][(N + 255) / 256] node-block partials (mm_cg_*)
+  double* cgpart = nullptr;           // [8][(N + 255) / 256] node-block partials (mm_cg_*)
+  double* cghost = nullptr;           // [8] pinned host staging of mm_cg_* results
   int* nanflag = nullptr;
   int32_t nsys = 1;                   // independent systems sharing the object
   int32_t* blk_sys = nullptr;         // [W/256] system of every cell block

@@ -63,13 +63,17 @@ hipError_t launch_reduce(const ReduceList& list, int* nanflag, hipStream_t strea
 hipError_t launch_force(const ForceArgs& a, hipStream_t stream);
 // Energy (partial row 0) and virial (rows 1-9) items of a single-system cell pass with nb blocks.
 void add_force_totals(ReduceList& list, const ForceArgs& a, int nb);
-// Node gather that also writes the per-workgroup partials of sum_n gpos[n] . dir[n] into
-// dpart[(N + 255) / 256].
-hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, double* dpart,
-                             hipStream_t stream);
-// Line-search trial point over n values: d <- beta d - g when set_dir, then xa <- x + alpha d.
+// Node gather of a line-search trial: also the per-workgroup partials of sum_n gpos_n . dir_n,
+// sum_n |gpos_n|^2, sum_n gpos_n . gold_n and max_n |gpos_n|^2, rows 0-3 of dpart (row stride
+// (N + 255) / 256).
+hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, const double* gold,
+                             double* dpart, hipStream_t stream);
+// Line-search trial point over N nodes: d <- beta d - g when set_dir, then xa <- x + alpha d;
+// per-workgroup partials of sum_n |xa_n - x_n|^2 and max_n |xa_n - x_n|^2, rows 0-1 of part
+// (row stride (N + 255) / 256).
 hipError_t launch_cg_trial(const double* x, const double* g, double* d, double beta, int set_dir,
-                           double alpha, double* xa, int64_t n, hipStream_t stream);
+                           double alpha, double* xa, int64_t N, double* part,
+                           hipStream_t stream);
 // out[0..5) = sum_n |g_n|^2, sum_n g_n . go_n, sum_n |x_n - xo_n|^2, max_n |g_n|^2,
 // max_n |x_n - xo_n|^2 over N nodes of 3 values; part holds 5 (N + 255) / 256 doubles.
 hipError_t launch_cg_stats(const double* x, const double* xo, const double* g, const double* go,

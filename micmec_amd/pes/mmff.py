@@ -241,21 +241,24 @@ class DeviceField(object):
                                         _native.stream_ptr(stream))
         _native.check(rc, "mm_field_compute")
 
-    def cg_trial(self, x, g, d, beta, set_dir, alpha, rvecs, xa, ga, out, stream=None):
-        """One line-search trial (asynchronous): with ``set_dir`` first d <- beta d - g, then
-        xa <- x + alpha d, energy and gradient ga at xa; out[0] = energy, out[1] = ga . d."""
+    def cg_trial(self, x, g, d, beta, set_dir, alpha, rvecs, xa, ga, out, host_out=None,
+                 stream=None):
+        """One line-search trial: with ``set_dir`` first d <- beta d - g, then xa <- x + alpha d,
+        energy and gradient ga at xa; out[:7] = energy, ga.d, ga.ga, ga.g, max_n |ga_n|^2,
+        sum_n |xa_n - x_n|^2, max_n |xa_n - x_n|^2. Asynchronous unless ``host_out`` (a ctypes
+        array of 7 doubles) is given, which then receives the results."""
         rc = self._lib.mm_cg_trial(self._handle, _native.ptr(x), _native.ptr(g), _native.ptr(d),
                                    float(beta), int(bool(set_dir)), float(alpha),
                                    _native.ptr(rvecs), _native.ptr(xa), _native.ptr(ga),
-                                   _native.ptr(out), _native.stream_ptr(stream))
+                                   _native.ptr(out), host_out, _native.stream_ptr(stream))
         _native.check(rc, "mm_cg_trial")
 
-    def cg_stats(self, x, x_old, g, g_old, out, stream=None):
+    def cg_stats(self, x, x_old, g, g_old, out, host_out=None, stream=None):
         """out[:5] = sum |g_n|^2, sum g_n . g_old_n, sum |x_n - x_old_n|^2, max |g_n|^2,
-        max |x_n - x_old_n|^2 over the nodes (asynchronous)."""
+        max |x_n - x_old_n|^2 over the nodes; ``host_out`` as for ``cg_trial``."""
         rc = self._lib.mm_cg_stats(self._handle, _native.ptr(x), _native.ptr(x_old),
                                    _native.ptr(g), _native.ptr(g_old), _native.ptr(out),
-                                   _native.stream_ptr(stream))
+                                   host_out, _native.stream_ptr(stream))
         _native.check(rc, "mm_cg_stats")
 
     def check(self, stream=None):

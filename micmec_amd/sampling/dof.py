@@ -10,6 +10,8 @@ gradient (and the system's positions) are refreshed at convergence, before host 
 the end of a run (``sync_from_device``), not after every iteration.
 """
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -61,13 +63,14 @@ class _DeviceCGEngine(object):
     """Device vectors of a conjugate-gradient search over all node positions of a
     ``CartesianDOF`` (the engine protocol of ``sampling.opt.VectorEngine``).
 
-    A trial point is one ``mm_cg_trial`` sequence: the pending direction update
-    d <- beta d - g and the point x + a d in one kernel, the force kernels, the slope g.d reduced
-    inside the node gather, then one read of two doubles. An accepted step is one
-    ``mm_cg_stats`` sequence whose sums and maxima give both the Polak-Ribiere coefficient and
-    the gpos/dpos measures of ``check_convergence`` (dpos against the previous accepted point,
-    which is where the last check left ``_last_pos``). Trial points alternate between two
-    buffers, so no vector is copied.
+    A trial point is one ``mm_cg_trial`` call: the pending direction update d <- beta d - g and
+    the point x + a d in one kernel, the force kernels, the slope g'.d and the measures an
+    accepted step needs (g'.g', g'.g, max |g'_n|^2, and the displacement from the current point,
+    which is where the last convergence check left ``_last_pos``) reduced inside the node gather
+    and the trial kernel, then one synchronising read of seven doubles. Accepting a trial
+    launches nothing: it swaps buffers and keeps the trial's measures for the Polak-Ribiere
+    coefficient and ``check_convergence``. Trial points alternate between two buffers, so no
+    vector is copied.
     """
 
     def __init__(self, dof, x0):
@@ -80,6 +83,8 @@ class _DeviceCGEngine(object):
         self.d = torch.zeros(n, **kw)
         self._slots = [[torch.empty(n, **kw), torch.empty(n, **kw)] for _ in range(2)]
         self._out = torch.zeros(8, **kw)
+        self._host = (ctypes.c_double * 8)()
+        self._stats = [None, None]
         self._last = None
         self._best = None
         self._beta = 0.0
@@ -89,9 +94,6 @@ class _DeviceCGEngine(object):
         self._measures = None
         self._checked = False
 
-    def _read(self, k):
-        return self._out[:k].tolist()
-
     def _energy(self, e):
         if np.isnan(e):
             raise ValueError("The energy is not-a-number (``nan``).")
@@ -100,12 +102,17 @@ class _DeviceCGEngine(object):
             e += press.pext * self.energy.fpm.system.domain.volume
         return e
 
+    def _measure(self, x, x_old, g, g_old):
+        """(sum |g_n|^2, sum g_n . g_old_n, sum |dx_n|^2, max |g_n|^2, max |dx_n|^2)."""
+        self.field.cg_stats(x, x_old, g, g_old, self._out, host_out=self._host)
+        return tuple(self._host[:5])
+
     def start(self):
         en = self.energy
         en.calls += 1
-        en.fpm.compute_device(self.x.view(-1, 3), en.rvecs, self._out[0:1], self.g.view(-1, 3))
-        self.field.cg_stats(self.x, self.x, self.g, self.g, self._out[1:6])
-        e, self.gg = self._read(2)
+        en.fpm.compute_device(self.x.view(-1, 3), en.rvecs, self._out[7:8], self.g.view(-1, 3))
+        e = float(self._out[7].item())
+        self.gg = self._measure(self.x, self.x, self.g, self.g)[0]
         self._beta, self._set_dir, self._dp = 0.0, True, 0.0
         self._measures, self._checked, self._best = None, False, None
         return self._energy(e), self.gg
@@ -125,9 +132,10 @@ class _DeviceCGEngine(object):
         en = self.energy
         en.calls += 1
         self.field.cg_trial(self.x, self.g, self.d, self._beta, self._set_dir, a, en.rvecs, xa,
-                            ga, self._out)
+                            ga, self._out, host_out=self._host)
         self._set_dir = False
-        e, dp = self._read(2)
+        e, dp, gg, g_gold, gsq_max, dsq, dsq_max = self._host[:7]
+        self._stats[k] = (gg, g_gold, gsq_max, dsq, dsq_max)
         self._last = k
         return self._energy(e), dp
 
@@ -136,15 +144,14 @@ class _DeviceCGEngine(object):
 
     def accept(self):
         k = self._best
+        gg, g_gold, gsq_max, dsq, dsq_max = self._stats[k]
         xa, ga = self._slots[k]
-        self.field.cg_stats(xa, self.x, ga, self.g, self._out)
-        gsq, g_gold, dsq, gsq_max, dsq_max = self._read(5)
         self._slots[k] = [self.x, self.g]
         self.x, self.g = xa, ga
         self._best = None
-        self._measures = (gsq, gsq_max, dsq, dsq_max)
-        self.gg = gsq
-        return gsq, gsq - g_gold
+        self._measures = (gg, gsq_max, dsq, dsq_max)
+        self.gg = gg
+        return gg, gg - g_gold
 
     def set_direction(self, beta, dp):
         self._beta, self._set_dir, self._dp = beta, True, dp
@@ -159,8 +166,7 @@ class _DeviceCGEngine(object):
         m = self._measures
         if m is None:
             # no step since the last check: the displacement is zero
-            self.field.cg_stats(self.x, self.x, self.g, self.g, self._out)
-            gsq, _, _, gsq_max, _ = self._read(5)
+            gsq, _, _, gsq_max, _ = self._measure(self.x, self.x, self.g, self.g)
             m = (gsq, gsq_max, 0.0, 0.0)
         self._measures = None
         return m

@@ -45,5 +45,6 @@ def test_argument_validation_without_gpu():
     assert b"model" in lib.mm_last_error()
     assert lib.mm_field_compute(None, None, None, None, None, None, 0, None) == _native.MM_ERR_ARG
     assert lib.mm_cg_trial(None, None, None, None, 0.0, 0, 0.0, None, None, None, None,
+                           None, None) == _native.MM_ERR_ARG
+    assert lib.mm_cg_stats(None, None, None, None, None, None, None,
                            None) == _native.MM_ERR_ARG
-    assert lib.mm_cg_stats(None, None, None, None, None, None, None) == _native.MM_ERR_ARG

@@ -7,6 +7,8 @@ with torch.dot); the statistics against torch reductions; the engine against the
 vector path of the same optimiser, and its convergence measures against the reference's
 formulas (dof.py:165-181) evaluated on the host copies of the accepted points."""
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -49,22 +51,30 @@ def test_cg_trial_equals_the_separate_steps(model, set_dir, beta):
     d_run = d.clone()
     xa = torch.empty(n, dtype=torch.float64, device=dev)
     ga = torch.empty(n, dtype=torch.float64, device=dev)
-    out = torch.zeros(2, dtype=torch.float64, device=dev)
-    fpm.field.cg_trial(x, g, d_run, beta, set_dir, alpha, rvecs, xa, ga, out)
+    out = torch.zeros(7, dtype=torch.float64, device=dev)
+    host = (ctypes.c_double * 7)()
+    fpm.field.cg_trial(x, g, d_run, beta, set_dir, alpha, rvecs, xa, ga, out, host_out=host)
     # the same force kernels at the trial point the kernel wrote
     e_ref = torch.zeros(1, dtype=torch.float64, device=dev)
     g_ref = torch.zeros((s.nnodes, 3), dtype=torch.float64, device=dev)
     fpm.compute_device(xa.view(-1, 3), rvecs, e_ref, g_ref)
     torch.cuda.synchronize()
+    got = out.tolist()
+    assert list(host) == got
     # direction and trial point to rounding (the kernel may fuse the multiply-add)
     assert torch.allclose(d_run, d_ref, rtol=0, atol=1e-15 * float(d_ref.abs().max()))
     assert torch.allclose(xa, x + alpha * d_run, rtol=0, atol=1e-15 * float(x.abs().max()))
     # energy and gradient: the same kernels and summation order as mm_field_compute
-    assert float(out[0]) == float(e_ref[0])
+    assert got[0] == float(e_ref[0])
     assert torch.equal(ga, g_ref.view(-1))
-    dot = torch.dot(ga, d_run)
-    scale = float(torch.dot(ga.abs(), d_run.abs()))
-    assert abs(float(out[1]) - float(dot)) <= 1e-13 * scale
+    # the reductions, against torch in another order
+    gsq = (ga.view(-1, 3) ** 2).sum(dim=1)
+    dsq = ((xa - x).view(-1, 3) ** 2).sum(dim=1)
+    ref = [torch.dot(ga, d_run), gsq.sum(), torch.dot(ga, g), gsq.max(), dsq.sum(), dsq.max()]
+    scale = [torch.dot(ga.abs(), d_run.abs()), gsq.sum(), torch.dot(ga.abs(), g.abs()),
+             gsq.max(), dsq.sum(), dsq.max()]
+    for q in range(6):
+        assert abs(got[1 + q] - float(ref[q])) <= 1e-13 * float(scale[q]), q
 
 
 def test_cg_stats_equal_torch_reductions():
@@ -74,8 +84,10 @@ def test_cg_stats_equal_torch_reductions():
     x, xo, g, go = (randn(n, k, sc, dev) for k, sc in ((1, 10.0), (2, 10.0), (3, 1e-3),
                                                      (4, 1e-3)))
     out = torch.zeros(5, dtype=torch.float64, device=dev)
-    fpm.field.cg_stats(x, xo, g, go, out)
+    host = (ctypes.c_double * 5)()
+    fpm.field.cg_stats(x, xo, g, go, out, host_out=host)
     torch.cuda.synchronize()
+    assert list(host) == out.tolist()
     gsq = (g.view(-1, 3) ** 2).sum(dim=1)
     dsq = ((x - xo).view(-1, 3) ** 2).sum(dim=1)
     ref = [gsq.sum(), torch.dot(g, go), dsq.sum(), gsq.max(), dsq.max()]
