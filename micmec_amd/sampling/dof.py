@@ -15,6 +15,8 @@ import ctypes
 import numpy as np
 import torch
 
+from .. import _native
+
 __all__ = ["DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF", "StrainCellDOF"]
 
 
@@ -84,6 +86,14 @@ class _DeviceCGEngine(object):
         self._slots = [[torch.empty(n, **kw), torch.empty(n, **kw)] for _ in range(2)]
         self._out = torch.zeros(8, **kw)
         self._host = (ctypes.c_double * 8)()
+        # raw arguments of the library calls, made once (the buffers never move)
+        self._lib = _native.lib()
+        self._check = _native.check
+        self._handle = self.field.handle
+        self._stream = ctypes.c_void_p(torch.cuda.current_stream(x0.device).cuda_stream)
+        self._ptr = {id(t): ctypes.c_void_p(t.data_ptr())
+                     for t in [self.x, self.g, self.d, self._out, self.energy.rvecs] +
+                     [t for slot in self._slots for t in slot]}
         self._stats = [None, None]
         self._last = None
         self._best = None
@@ -131,8 +141,12 @@ class _DeviceCGEngine(object):
         xa, ga = self._slots[k]
         en = self.energy
         en.calls += 1
-        self.field.cg_trial(self.x, self.g, self.d, self._beta, self._set_dir, a, en.rvecs, xa,
-                            ga, self._out, host_out=self._host)
+        P = self._ptr
+        rc = self._lib.mm_cg_trial(self._handle, P[id(self.x)], P[id(self.g)], P[id(self.d)],
+                                   self._beta, int(self._set_dir), a, P[id(en.rvecs)], P[id(xa)],
+                                   P[id(ga)], P[id(self._out)], self._host, self._stream)
+        if rc:
+            self._check(rc, "mm_cg_trial")
         self._set_dir = False
         e, dp, gg, g_gold, gsq_max, dsq, dsq_max = self._host[:7]
         self._stats[k] = (gg, g_gold, gsq_max, dsq, dsq_max)

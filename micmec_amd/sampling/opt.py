@@ -10,6 +10,7 @@ one launch of the HIP force kernels; only scalars return to the host.
 """
 
 import time
+from math import isfinite
 
 import numpy as np
 import torch
@@ -189,7 +190,7 @@ class ConjugateGradientSearch(object):
         best = None
         for _ in range(self.max_ls):
             fa, dp = E.trial(a)
-            ok = np.isfinite(fa) and np.isfinite(dp) and fa <= f0 + ftol
+            ok = isfinite(fa) and isfinite(dp) and fa <= f0 + ftol
             if ok and (best is None or abs(dp) < abs(best[2])):
                 best = (a, fa, dp)
                 E.keep()
@@ -199,7 +200,7 @@ class ConjugateGradientSearch(object):
                 prev_a, prev_dp = lo_a, lo_dp
                 lo_a, lo_dp = a, dp
             else:
-                hi_a, hi_dp = a, (dp if ok or (np.isfinite(fa) and np.isfinite(dp)) else None)
+                hi_a, hi_dp = a, (dp if ok or (isfinite(fa) and isfinite(dp)) else None)
             if hi_a is None:
                 # still descending: extrapolate the secant of phi' through the last two points
                 # (beyond a when phi' rises), at most ten times as far
@@ -212,7 +213,7 @@ class ConjugateGradientSearch(object):
                 else:
                     a_new = 0.5 * (lo_a + hi_a)
                 width = hi_a - lo_a
-                if not np.isfinite(a_new):
+                if not isfinite(a_new):
                     a_new = lo_a + 0.5 * width
                 a_new = min(max(a_new, lo_a + 0.01 * width), hi_a - 0.01 * width)
             if self.max_step is not None:
