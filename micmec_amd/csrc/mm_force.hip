@@ -155,7 +155,8 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
 #pragma unroll
       for (int v = 0; v < 8; ++v)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) gcell[(int64_t)(3 * v + j) * W + w] = gacc[v][j];
+        for (int j = 0; j < 3; ++j)
+          __builtin_nontemporal_store(gacc[v][j], &gcell[(int64_t)(3 * v + j) * W + w]);
       // virial: vtens[k][l] = sum_v g_v[k] (r_v - r_0)[l]
 #pragma unroll
       for (int v = 1; v < 8; ++v) {
@@ -244,7 +245,8 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
         const int64_t w = e >> 3;
         const int kk = e & 7;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
+        for (int j = 0; j < 3; ++j)
+          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * kk + j) * W + w]);
       }
     }
   }

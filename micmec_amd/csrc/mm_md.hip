@@ -252,7 +252,8 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
         const int64_t w = e >> 3;
         const int kk = e & 7;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) g[j] += gcell[(int64_t)(3 * kk + j) * W + w];
+        for (int j = 0; j < 3; ++j)
+          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * kk + j) * W + w]);
       }
     }
 #pragma unroll
@@ -298,65 +299,6 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
 
 // ----------------------------------------------------------------------------- scalar math
 
-struct Sym3 {
-  double a[9];
-};
-
-// Cyclic Jacobi eigendecomposition of a symmetric 3x3 matrix: A = Q diag(w) Q^T.
-__device__ void jacobi3(const double* A_in, double w[3], double Q[9]) {
-  double A[9];
-  for (int i = 0; i < 9; ++i) A[i] = A_in[i];
-  for (int i = 0; i < 9; ++i) Q[i] = (i % 4 == 0) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 50; ++sweep) {
-    const double off = fabs(A[1]) + fabs(A[2]) + fabs(A[5]);
-    const double scale = fabs(A[0]) + fabs(A[4]) + fabs(A[8]) + off;
-    if (off == 0.0 || off <= 1e-300 + 1e-18 * scale) break;
-    for (int p = 0; p < 2; ++p) {
-      for (int q = p + 1; q < 3; ++q) {
-        const double apq = A[3 * p + q];
-        if (apq == 0.0) continue;
-        const double app = A[3 * p + p], aqq = A[3 * q + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0.0 ? 1.0 : -1.0) /
-                         (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        // A <- J^T A J with J the rotation in the (p, q) plane
-        for (int k = 0; k < 3; ++k) {
-          const double akp = A[3 * k + p], akq = A[3 * k + q];
-          A[3 * k + p] = c * akp - s * akq;
-          A[3 * k + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < 3; ++k) {
-          const double apk = A[3 * p + k], aqk = A[3 * q + k];
-          A[3 * p + k] = c * apk - s * aqk;
-          A[3 * q + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < 3; ++k) {
-          const double qkp = Q[3 * k + p], qkq = Q[3 * k + q];
-          Q[3 * k + p] = c * qkp - s * qkq;
-          Q[3 * k + q] = s * qkp + c * qkq;
-        }
-      }
-    }
-  }
-  w[0] = A[0];
-  w[1] = A[4];
-  w[2] = A[8];
-}
-
-// R = Q diag(exp((sign * w) * dt / 2)) Q^T for symmetric A (npt.py:686-690, 709-726).
-__device__ void expm_sym_half(const double* A, double sign, double dt, double* R) {
-#pragma clang fp contract(off)
-  double w[3], Q[9];
-  jacobi3(A, w, Q);
-  double e[3];
-  for (int i = 0; i < 3; ++i) e[i] = exp(((sign * w[i]) * dt) / 2.0);
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      R[3 * i + j] = ((Q[3 * i + 0] * e[0]) * Q[3 * j + 0] + (Q[3 * i + 1] * e[1]) * Q[3 * j + 1]) +
-                     (Q[3 * i + 2] * e[2]) * Q[3 * j + 2];
-}
-
 __device__ void matmul3(const double* A, const double* B, double* C) {
 #pragma clang fp contract(off)
   double T[9];
@@ -364,6 +306,55 @@ __device__ void matmul3(const double* A, const double* B, double* C) {
     for (int j = 0; j < 3; ++j)
       T[3 * i + j] = (A[3 * i + 0] * B[0 + j] + A[3 * i + 1] * B[3 + j]) + A[3 * i + 2] * B[6 + j];
   for (int i = 0; i < 9; ++i) C[i] = T[i];
+}
+
+// R = exp(sign * A * dt / 2) for symmetric A: the matrix the reference forms from
+// eigh(A) = (w, Q) as Q diag(exp(sign w dt / 2)) Q^T (npt.py:686-690, 709-726), here by scaling
+// and squaring. X = sign * A * dt / 2 is halved s times until ||X||_inf <= 1/2, exp(X) is the
+// Taylor polynomial to the X^14 term in Horner form (remainder below 0.5^15 / 15! < 3e-17
+// relative), and the result is squared s times. Multiply-adds only: no square roots or
+// divisions on the serial path.
+__device__ void expm_sym_half(const double* A, double sign, double dt, double* R) {
+  const double h = (sign * dt) / 2.0;
+  double X[9];
+  double nrm = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double r = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      X[3 * i + j] = h * A[3 * i + j];
+      r += fabs(X[3 * i + j]);
+    }
+    nrm = fmax(nrm, r);
+  }
+  int s = 0;
+  double scale = 1.0;
+  while (nrm * scale > 0.5 && s < 64) {
+    scale *= 0.5;
+    ++s;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) X[q] *= scale;
+  double P[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) P[q] = (q % 4 == 0) ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 14; k >= 1; --k) {
+    // P <- I + X P / k
+    const double inv = 1.0 / (double)k;
+    double T[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        T[3 * i + j] = X[3 * i + 0] * P[0 + j] + X[3 * i + 1] * P[3 + j] + X[3 * i + 2] * P[6 + j];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) P[q] = T[q] * inv + ((q % 4 == 0) ? 1.0 : 0.0);
+  }
+  for (int i = 0; i < s; ++i) matmul3(P, P, P);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) R[q] = P[q];
 }
 
 // |det| of the domain vectors, as micmec/pes/domain.c:42-48.
@@ -886,10 +877,14 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
     __syncthreads();
     if (!is_last) return;
     __threadfence();
+    // all slices into LDS in one parallel pass, then each total summed in slice order
+    __shared__ double sl_sh[kScalarSlices * kSliceQ];
+    const double* sl = slices + (int64_t)sys * G * kSliceQ;
+    for (int i = threadIdx.x; i < G * kSliceQ; i += kScalarThreads) sl_sh[i] = sl[i];
+    __syncthreads();
     if (threadIdx.x < kSliceQ) {
-      const double* sl = slices + (int64_t)sys * G * kSliceQ;
       double t = 0.0;
-      for (int k = 0; k < G; ++k) t += sl[k * kSliceQ + threadIdx.x];
+      for (int k = 0; k < G; ++k) t += sl_sh[k * kSliceQ + threadIdx.x];
       tot[threadIdx.x] = t;
     }
     if (threadIdx.x == 0) counters[sys] = 0u;
