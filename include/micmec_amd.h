@@ -140,6 +140,16 @@ int mm_field_set_params(mm_field* f, const int32_t* nstates, const double* h0,
                         const double* cvoigt, const double* free_energy, const double* temp_eff,
                         double boltzmann, void* stream);
 
+/* mm_field_set_params with h0, cvoigt, free_energy and temp_eff in device memory (nstates stays
+ * a host array): a kernel on `stream` builds the table with the host builder's arithmetic (the
+ * same bits), so nothing synchronises unless nstates differs from the object's current counts.
+ * Later work must be ordered after `stream`. A singular or non-finite h0 is not reported here:
+ * it gives non-finite table entries and then non-finite energies (mm_field_check). No
+ * reference counterpart (parameter fitting, as mm_field_set_params). */
+int mm_field_set_params_device(mm_field* f, const int32_t* nstates, const double* h0,
+                               const double* cvoigt, const double* free_energy,
+                               const double* temp_eff, double boltzmann, void* stream);
+
 /* ------------------------------------------------------------------------------------------
  * Differentiation form. No reference counterpart for the parameter derivatives; the position
  * and deformation derivatives agree with ForcePartMechanical (micmec/pes/mmff.py:288-323) and

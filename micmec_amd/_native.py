@@ -51,6 +51,7 @@ def _load():
     L.mm_field_launch_cells.argtypes = [P, P, P, P]
     L.mm_cg_trial.argtypes = [P, P, P, P, f64, ctypes.c_int, f64, P, P, P, P, P, P]
     L.mm_cg_stats.argtypes = [P, P, P, P, P, P, P, P]
+    L.mm_field_set_params_device.argtypes = [P, P, P, P, P, P, f64, P]
     L.mm_field_info.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.mm_field_destroy.argtypes = [P]
     L.mm_topology_build.argtypes = [ctypes.POINTER(P), ctypes.c_int, P, i32, i32, i32, i32, P,

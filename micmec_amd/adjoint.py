@@ -94,6 +94,20 @@ def tensor_from_voigt(cv):
     return out
 
 
+def _inv3_det(h):
+    """Inverses and determinants of [..., 3, 3] matrices by cofactors (elementwise tensor
+    arithmetic: no library solve and no synchronisation)."""
+    a, b, c = h[..., 0, 0], h[..., 0, 1], h[..., 0, 2]
+    d, e, f = h[..., 1, 0], h[..., 1, 1], h[..., 1, 2]
+    g, k, l = h[..., 2, 0], h[..., 2, 1], h[..., 2, 2]
+    c00, c01, c02 = e * l - f * k, f * g - d * l, d * k - e * g
+    det = a * c00 + b * c01 + c * c02
+    rows = [torch.stack([c00, c * k - b * l, b * f - c * e], -1),
+            torch.stack([c01, a * l - c * g, c * d - a * f], -1),
+            torch.stack([c02, b * g - a * k, a * e - b * d], -1)]
+    return torch.stack(rows, -2) / det[..., None, None], det
+
+
 class CellParams(object):
     """Differentiable cell-type parameters (float64 tensors) and the fixed state counts."""
 
@@ -187,16 +201,23 @@ class DiffField(object):
         return CellParams.from_table(self.table, self.device, requires_grad)
 
     def set_params(self, params):
-        """Upload ``params`` (a CellParams) as the object's parameter table (synchronises the
-        current stream; the table is small)."""
-        hp = [np.ascontiguousarray(x.detach().cpu().numpy(), dtype=np.float64)
-              for x in params.tensors()]
+        """Make ``params`` (a CellParams) the object's parameter table: a kernel on the current
+        stream builds it from the device tensors, so nothing synchronises. Uploading the same,
+        unmodified tensors again (the backward pass after its forward) is skipped."""
+        ts = params.tensors()
+        key = (tuple(ts), tuple(x._version for x in ts), params.nstates.tobytes())
+        last = getattr(self, "_uploaded", None)
+        if last is not None and len(last[0]) == len(ts) and \
+                all(a is b for a, b in zip(last[0], key[0])) and last[1:] == key[1:]:
+            return
+        kw = dict(dtype=torch.float64, device=self.device)
+        cv, h0, free, teff = [x.detach().to(**kw).contiguous() for x in ts]
         ns = np.ascontiguousarray(params.nstates, dtype=np.int32)
-        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-        rc = self.lib.mm_field_set_params(self._field, vp(ns), vp(hp[1]), vp(hp[0]), vp(hp[2]),
-                                          vp(hp[3]), float(units.boltzmann),
-                                          _native.stream_ptr())
-        _native.check(rc, "mm_field_set_params")
+        rc = self.lib.mm_field_set_params_device(
+            self._field, ns.ctypes.data_as(ctypes.c_void_p), _native.ptr(h0), _native.ptr(cv),
+            _native.ptr(free), _native.ptr(teff), float(units.boltzmann), _native.stream_ptr())
+        _native.check(rc, "mm_field_set_params_device")
+        self._uploaded = key
 
     # ------------------------------------------------------------------ kernels
     def compute(self, pos, rvecs):
@@ -260,19 +281,26 @@ class DiffField(object):
             g_cv[:, :, J, I] = ent[:, :, q]
         gP = ent[:, :, 21:30].reshape(T, S, 3, 3)
         gw = ent[:, :, 30]
-        h0n = h0.cpu().numpy()
-        kw = dict(dtype=torch.float64, device=self.device)
-        Pt = torch.tensor(np.swapaxes(np.linalg.inv(h0n), -1, -2), **kw)
-        det = torch.tensor(np.linalg.det(h0n), **kw)
+        inv, det = _inv3_det(h0)
+        Pt = inv.transpose(-1, -2)
         w = det / 8.0 if self.model == "corner8" else det
         g_h0 = -(Pt @ gP @ Pt) + (gw * w)[..., None, None] * Pt
         g_free = ent[:, :, 31].clone()
         g_teff = entries[:, S * NPS] * units.boltzmann
-        mask = torch.tensor(np.arange(S)[None, :] < params.nstates[:, None], device=self.device)
+        mask = self._state_mask(params.nstates)
         g_cv = g_cv * mask[..., None, None]
         g_h0 = g_h0 * mask[..., None, None]
         g_free = g_free * mask
         return g_cv, g_h0, g_free, g_teff
+
+    def _state_mask(self, nstates):
+        """[nrows, max_states] device mask of the states each row has (cached)."""
+        key = np.asarray(nstates, dtype=np.int32).tobytes()
+        cache = self.__dict__.setdefault("_masks", {})
+        if key not in cache:
+            cache[key] = torch.tensor(np.arange(self.max_states)[None, :] <
+                                      np.asarray(nstates)[:, None], device=self.device)
+        return cache[key]
 
     def __del__(self):
         lib = getattr(self, "lib", None)
@@ -408,9 +436,12 @@ def fit(problem, init, steps, lr_cv=1e-2, lr_h0=1e-4, callback=None):
         loss = problem.loss(params_of(a, b))
         loss.backward()
         opt.step()
-        losses.append(float(loss.detach()))
+        # the history stays on the device (no synchronisation per step) unless a callback
+        # wants the value now
+        losses.append(loss.detach())
         if callback is not None:
-            callback(it, losses[-1])
+            callback(it, float(losses[-1]))
     with torch.no_grad():
         final = params_of(a, b)
+    losses = torch.stack(losses).cpu().tolist() if losses else []
     return losses, final

@@ -1,6 +1,7 @@
 // C ABI: device-resident force field objects (see include/micmec_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -33,23 +34,7 @@ int hip_error(hipError_t e, const char* what) {
   return set_error(MM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-// 3x3 inverse (row-major) by cofactors; returns the determinant.
-double inverse3(const double* h, double* inv) {
-  const double a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], k = h[7],
-               l = h[8];
-  const double c00 = e * l - f * k, c01 = f * g - d * l, c02 = d * k - e * g;
-  const double det = a * c00 + b * c01 + c * c02;
-  inv[0] = c00 / det;
-  inv[1] = (c * k - b * l) / det;
-  inv[2] = (b * f - c * e) / det;
-  inv[3] = c01 / det;
-  inv[4] = (a * l - c * g) / det;
-  inv[5] = (c * d - a * f) / det;
-  inv[6] = c02 / det;
-  inv[7] = (b * g - a * k) / det;
-  inv[8] = (a * e - b * d) / det;
-  return det;
-}
+double inverse3(const double* h, double* inv) { return inverse3_cof(h, inv); }
 
 // Voigt pair (i, j) of index I: 0:(0,0) 1:(1,1) 2:(2,2) 3:(1,2) 4:(0,2) 5:(0,1)
 // (micmec/analysis/tensor.py:93).
@@ -358,6 +343,7 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
     return hip_error(e, "mm_field_create");
   }
   f->blk_row_host = blk_row;
+  f->nstates_host.assign(nstates, nstates + nrows);
   update_split(f, nstates);
   *out = f;
   return MM_OK;
@@ -409,7 +395,35 @@ int mm_field_set_params(mm_field* f, const int32_t* nstates, const double* h0,
   if (e == hipSuccess)
     e = hipMemcpy(f->nstates, nstates, sizeof(int32_t) * f->nrows, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_error(e, "mm_field_set_params");
+  f->nstates_host.assign(nstates, nstates + f->nrows);
   update_split(f, nstates);
+  return MM_OK;
+}
+
+int mm_field_set_params_device(mm_field* f, const int32_t* nstates, const double* h0,
+                               const double* cvoigt, const double* free_energy,
+                               const double* temp_eff, double boltzmann, void* stream) {
+  if (!f || !nstates || !h0 || !cvoigt || !free_energy || !temp_eff)
+    return set_error(MM_ERR_ARG, "missing argument");
+  for (int t = 0; t < f->nrows; ++t)
+    if (nstates[t] < 1 || nstates[t] > f->max_states)
+      return set_error(MM_ERR_ARG, "row %d has %d states (1..%d allowed)", t, nstates[t],
+                       f->max_states);
+  DeviceGuard guard(f->device);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (!std::equal(f->nstates_host.begin(), f->nstates_host.end(), nstates)) {
+    // a new state count per row: the work order's split changes too (rare; synchronises)
+    e = hipStreamSynchronize(s);
+    if (e == hipSuccess)
+      e = hipMemcpy(f->nstates, nstates, sizeof(int32_t) * f->nrows, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_error(e, "mm_field_set_params_device");
+    f->nstates_host.assign(nstates, nstates + f->nrows);
+    update_split(f, nstates);
+  }
+  e = launch_build_table(f->model == MM_MODEL_CORNER8, f->nrows, f->max_states, f->nstates, h0,
+                         cvoigt, free_energy, temp_eff, boltzmann, f->tab, f->kt, s);
+  if (e != hipSuccess) return hip_error(e, "parameter table kernel");
   return MM_OK;
 }
 

@@ -26,6 +26,26 @@ constexpr int kTabStride = 84;
 constexpr int kCvOffset = 12;
 constexpr int kCvwOffset = 48;
 
+// 3x3 inverse (row-major) by cofactors; returns the determinant. Shared by the host and the
+// device table builders, without contraction, so both round the same way.
+__host__ __device__ inline double inverse3_cof(const double* h, double* inv) {
+#pragma clang fp contract(off)
+  const double a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], k = h[7],
+               l = h[8];
+  const double c00 = e * l - f * k, c01 = f * g - d * l, c02 = d * k - e * g;
+  const double det = a * c00 + b * c01 + c * c02;
+  inv[0] = c00 / det;
+  inv[1] = (c * k - b * l) / det;
+  inv[2] = (b * f - c * e) / det;
+  inv[3] = c01 / det;
+  inv[4] = (a * l - c * g) / det;
+  inv[5] = (c * d - a * f) / det;
+  inv[6] = c02 / det;
+  inv[7] = (b * g - a * k) / det;
+  inv[8] = (a * e - b * d) / det;
+  return det;
+}
+
 // Edge endpoints: corner a, axis d -> lower / upper corner of the edge along d through a.
 __device__ constexpr int kLo[8][3] = {
     {0, 0, 0}, {0, 1, 1}, {2, 0, 2}, {3, 3, 0}, {2, 1, 4}, {3, 5, 1}, {6, 3, 2}, {6, 5, 4}};

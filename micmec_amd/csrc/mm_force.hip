@@ -240,7 +240,7 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
   if (n < N) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int32_t e = ncell[k * N + n];
+      const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
       if (e >= 0) {
         const int64_t w = e >> 3;
         const int kk = e & 7;
@@ -483,6 +483,52 @@ hipError_t launch_cg_stats(const double* x, const double* xo, const double* g, c
   for (int q = 0; q < 5; ++q)
     list.add(part + (int64_t)q * nb, nb, q < 3 ? kReduceSum : kReduceMax, out + q, 0, 0);
   return launch_reduce(list, nanflag, stream);
+}
+
+// Parameter table entries (row t, state s) from device arrays, one thread per entry, with the
+// arithmetic of the host builder (Voigt input), so both give the same bits; entries of states
+// beyond nstates[t] are zero. kt[t] = boltzmann * temp_eff[t].
+__global__ void __launch_bounds__(64)
+build_table_kernel(int corner8, int nrows, int max_states, const int32_t* __restrict__ nstates,
+                   const double* __restrict__ h0, const double* __restrict__ cv,
+                   const double* __restrict__ free_energy, const double* __restrict__ temp_eff,
+                   double boltzmann, double* __restrict__ tab, double* __restrict__ kt) {
+#pragma clang fp contract(off)
+  const int ts = (int)blockIdx.x * 64 + threadIdx.x;
+  if (ts >= nrows * max_states) return;
+  const int t = ts / max_states, s = ts % max_states;
+  double* e = tab + (int64_t)ts * kTabStride;
+  if (s == 0) kt[t] = boltzmann * temp_eff[t];
+  if (s >= nstates[t]) {
+    for (int q = 0; q < kTabStride; ++q) e[q] = 0.0;
+    return;
+  }
+  double inv[9];
+  const double det = inverse3_cof(h0 + (int64_t)ts * 9, inv);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) e[q] = inv[q];
+  const double w = corner8 ? 0.125 * det : det;
+  e[9] = w;
+  e[10] = free_energy[ts];
+  e[11] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 36; ++q) {
+    const double c = cv[(int64_t)ts * 36 + q];
+    e[kCvOffset + q] = c;
+    e[kCvwOffset + q] = w * c;
+  }
+}
+
+hipError_t launch_build_table(int corner8, int nrows, int max_states, const int32_t* nstates,
+                              const double* h0, const double* cv, const double* free_energy,
+                              const double* temp_eff, double boltzmann, double* tab, double* kt,
+                              hipStream_t stream) {
+  const int n = nrows * max_states;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(build_table_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, corner8,
+                     nrows, max_states, nstates, h0, cv, free_energy, temp_eff, boltzmann, tab,
+                     kt);
+  return hipGetLastError();
 }
 
 // Per-system totals of a batch: one block per system sums its own cell blocks in list order.

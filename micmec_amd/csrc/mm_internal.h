@@ -43,6 +43,7 @@ struct mm_field {
   int32_t* sys_cblk_idx = nullptr;    // [W/256]
   std::vector<int64_t> slot_of_cell;  // reference cell index -> work slot
   std::vector<int32_t> blk_row_host;  // [W/256] copy of blk_row
+  std::vector<int32_t> nstates_host;  // [nrows] copy of nstates
   int split = -1;                     // see ForceArgs::split
   hipStream_t side = nullptr;         // multistate instance of the cell pass runs here
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

@@ -39,6 +39,13 @@ struct ForceArgs {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
+// Parameter table [nrows][max_states][kTabStride] and kt [nrows] from device arrays (h0 [.][9],
+// Voigt cv [.][36], free_energy, temp_eff; nstates device), as the host builder computes them.
+hipError_t launch_build_table(int corner8, int nrows, int max_states, const int32_t* nstates,
+                              const double* h0, const double* cv, const double* free_energy,
+                              const double* temp_eff, double boltzmann, double* tab, double* kt,
+                              hipStream_t stream);
+
 // Reductions of device arrays into single values, one workgroup per item (reduce_items_kernel).
 constexpr int kReduceSum = 0;
 constexpr int kReduceMax = 1;

@@ -247,7 +247,7 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     double g[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int32_t e = ncell[k * N + n];
+      const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
       if (e >= 0) {
         const int64_t w = e >> 3;
         const int kk = e & 7;

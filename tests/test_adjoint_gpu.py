@@ -346,3 +346,34 @@ def test_fit_reduces_loss():
     problem = FitProblem(field, pos, rv, target)
     losses, final = fit(problem, field.initial_params(), 60)
     assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
+
+
+def test_device_parameter_table_equals_host_table(setup):
+    """mm_field_set_params_device builds the table with the host builder's arithmetic: the
+    energies, gradients and virials are the same bits; an in-place change is uploaded again."""
+    import ctypes
+    from micmec_amd import _native
+    field, pos, rv, params = setup
+    p = params_copy(params)
+    gen = torch.Generator().manual_seed(5)
+    p.cv.mul_(1.0 + 0.05 * torch.rand(p.cv.shape, generator=gen, dtype=torch.float64)
+              .to(p.cv.device))
+    p.h0.mul_(1.0 + 0.01 * torch.rand(p.h0.shape, generator=gen, dtype=torch.float64)
+              .to(p.h0.device))
+    hp = [np.ascontiguousarray(x.detach().cpu().numpy(), dtype=np.float64) for x in p.tensors()]
+    ns = np.ascontiguousarray(p.nstates, dtype=np.int32)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = field.lib.mm_field_set_params(field._field, vp(ns), vp(hp[1]), vp(hp[0]), vp(hp[2]),
+                                       vp(hp[3]), float(units.boltzmann), _native.stream_ptr())
+    assert rc == 0
+    field._uploaded = None
+    e1, g1, v1 = field.compute(pos, rv)
+    field.set_params(p)
+    e2, g2, v2 = field.compute(pos, rv)
+    torch.cuda.synchronize()
+    assert torch.equal(e1, e2) and torch.equal(g1, g2) and torch.equal(v1, v2)
+    p.free.add_(1e-3)
+    field.set_params(p)
+    e3, _, _ = field.compute(pos, rv)
+    assert not torch.equal(e3, e2)
+    field.set_params(params)
