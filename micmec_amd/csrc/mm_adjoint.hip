@@ -359,8 +359,9 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
   if (valid) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      gcell[(int64_t)(3 * kref + j) * W + w] = val(g[j]);
-      if (IsDual<T>::value) gcell2[(int64_t)(3 * kref + j) * W + w] = tan_(g[j]);
+      __builtin_nontemporal_store(val(g[j]), &gcell[(int64_t)(3 * kref + j) * W + w]);
+      if (IsDual<T>::value)
+        __builtin_nontemporal_store(tan_(g[j]), &gcell2[(int64_t)(3 * kref + j) * W + w]);
     }
   }
   // energy, virial sum_v g_v (x) (r_v - r_0) and the domain-vector gradient: corner b moves
