@@ -20,6 +20,18 @@ from .. import _native
 __all__ = ["DOF", "CartesianDOF", "BaseCellDOF", "FullCellDOF", "StrainCellDOF"]
 
 
+def _pinned_like(t):
+    """A pinned host tensor of t's shape and dtype (for DMA copies from the device)."""
+    return torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+
+
+def _to_host(src, dst, stage):
+    """Copy the device tensor ``src`` into the numpy array ``dst`` through the pinned tensor
+    ``stage`` (one DMA transfer, then a host copy)."""
+    stage.copy_(src.detach())
+    dst[...] = stage.numpy().reshape(dst.shape)
+
+
 class _DeviceEnergy(object):
     """Energy and gradient of a device position tensor [N, 3]."""
 
@@ -267,6 +279,7 @@ class CartesianDOF(DOF):
                 not (isinstance(x0, torch.Tensor) and x0.is_cuda):
             return None
         self._cg_engine = _DeviceCGEngine(self, x0)
+        self._host_stage(self._cg_engine.x.view(-1, 3))  # pinned staging, made before the run
         return self._cg_engine
 
     def fun(self, x, do_gradient=False):
@@ -311,13 +324,19 @@ class CartesianDOF(DOF):
             else:
                 self._gpos[self.select] = np.asarray(g).reshape(-1, 3)
 
+    def _host_stage(self, like):
+        stage = getattr(self, "_stage", None)
+        if stage is None or stage.shape != like.shape or stage.dtype != like.dtype:
+            stage = self._stage = _pinned_like(like)
+        return stage
+
     def sync_from_device(self):
         """Copy the last device positions and gradient into the host arrays and the system."""
         if getattr(self, "_pos_dev", None) is not None:
-            self._pos[:] = self._pos_dev.cpu().numpy()
+            _to_host(self._pos_dev, self._pos, self._host_stage(self._pos_dev))
             self.mmf.update_pos(self._pos[:])
         if getattr(self, "_gpos_dev", None) is not None:
-            self._gpos[:] = self._gpos_dev.cpu().numpy()
+            _to_host(self._gpos_dev, self._gpos, self._host_stage(self._gpos_dev))
 
     def _device_measures(self):
         """gpos_max, gpos_rms, dpos_max, dpos_rms of the device state (one small copy)."""
@@ -555,9 +574,14 @@ class BaseCellDOF(DOF):
 
     def sync_from_device(self):
         if self._pos_dev is not None:
-            self._pos[:] = self._pos_dev.cpu().numpy()
+            stage = getattr(self, "_stage", None)
+            if stage is None or stage.shape != self._pos_dev.shape:
+                stage = self._stage = _pinned_like(self._pos_dev)
+            _to_host(self._pos_dev, self._pos, stage)
             self.mmf.update_pos(self._pos[:])
-        if self._gpos_dev is not None:
+            if self._gpos_dev is not None:
+                _to_host(self._gpos_dev, self._gpos, stage)
+        elif self._gpos_dev is not None:
             self._gpos[:] = self._gpos_dev.cpu().numpy()
 
     # -------------------------------------------------------------------------- convergence
