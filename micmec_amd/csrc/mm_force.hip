@@ -95,7 +95,9 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             int32_t max_states, int64_t W, double* __restrict__ gcell,
             double* __restrict__ partials, int blk0, int nblocks) {
   __shared__ double red[10 * (kCellBlock / 64)];
-  __shared__ double es_sh[kMulti ? kMaxStates * kCellBlock : 1];
+  __shared__ double es_sh[(kMulti && !kGrad) ? kMaxStates * kCellBlock : 1];
+  // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
+  __shared__ double gs_sh[(kMulti && kGrad) ? 24 * kCellBlock : 1];
   const int bid = blk0 + xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int trow = blk_row[bid];
   // a block of the other kind (full-range launches only) leaves its partials to the other
@@ -128,9 +130,9 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
     double ecell;
     if (!kMulti) {
       ecell = tab[10] + state_energy<kModel, kGrad, true>(tab, r, gacc);
-    } else {
-      // pass 1: energies of all states, E = E_min - kT ln sum exp(-(E_s - E_min)/kT)
-      // (mmff.py:386-395); the state energies are kept in LDS
+    } else if (!kGrad) {
+      // energies of all states, E = E_min - kT ln sum exp(-(E_s - E_min)/kT) (mmff.py:386-395);
+      // the state energies are kept in LDS
       double emin = 0.0;
       for (int s = 0; s < ns; ++s) {
         const double* ts = tab + s * kTabStride;
@@ -141,14 +143,43 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       double zsum = 0.0;
       for (int s = 0; s < ns; ++s) zsum += exp(-(es_sh[s * kCellBlock + threadIdx.x] - emin) / kt);
       ecell = emin - kt * log(zsum);
-      // pass 2: gradient sum_s p_s dE_s/dr with p_s = exp(-(E_s - E_min)/kT) / Z (mmff.py:396-398)
-      if (kGrad) {
-        for (int s = 0; s < ns; ++s) {
-          const double* ts = tab + s * kTabStride;
-          const double ps = exp(-(es_sh[s * kCellBlock + threadIdx.x] - emin) / kt) / zsum;
-          (void)state_energy<kModel, true>(ts, r, gacc, ps);
+    } else {
+      // one pass over the states: energy and gradient of each, and the gradient
+      // sum_s p_s dE_s/dr with p_s = exp(-(E_s - E_min)/kT) / Z (mmff.py:386-398) accumulated
+      // in LDS with weights relative to the lowest energy so far, rescaled when it moves
+      double* gsum = gs_sh + threadIdx.x;
+      double emin = 0.0, zsum = 0.0;
+      for (int s = 0; s < ns; ++s) {
+        const double* ts = tab + s * kTabStride;
+        double gs[8][3];
+#pragma unroll
+        for (int v = 0; v < 8; ++v)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) gs[v][j] = 0.0;
+        const double es = ts[10] + state_energy<kModel, true, true>(ts, r, gs);
+        if (s == 0) {
+          emin = es;
+          zsum = 1.0;
+#pragma unroll
+          for (int q = 0; q < 24; ++q) gsum[q * kCellBlock] = gs[q / 3][q % 3];
+        } else if (es < emin) {
+          const double c = exp(-(emin - es) / kt);
+          zsum = zsum * c + 1.0;
+#pragma unroll
+          for (int q = 0; q < 24; ++q)
+            gsum[q * kCellBlock] = gsum[q * kCellBlock] * c + gs[q / 3][q % 3];
+          emin = es;
+        } else {
+          const double c = exp(-(es - emin) / kt);
+          zsum += c;
+#pragma unroll
+          for (int q = 0; q < 24; ++q) gsum[q * kCellBlock] += c * gs[q / 3][q % 3];
         }
       }
+      ecell = emin - kt * log(zsum);
+      const double inv = 1.0 / zsum;
+#pragma unroll
+      for (int q = 0; q < 24; ++q) gacc[q / 3][q % 3] = gsum[q * kCellBlock] * inv;
     }
     acc[0] = ecell;
     if (kGrad) {
