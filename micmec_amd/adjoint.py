@@ -275,14 +275,12 @@ class DiffField(object):
         cv, h0, free, teff = [x.detach() for x in params.tensors()]
         T, S = self.nrows, self.max_states
         ent = entries[:, : S * NPS].reshape(T, S, NPS)
-        g_cv = torch.zeros((T, S, 6, 6), dtype=torch.float64, device=self.device)
-        for q, (I, J) in enumerate(voigt_pairs()):
-            g_cv[:, :, I, J] = ent[:, :, q]
-            g_cv[:, :, J, I] = ent[:, :, q]
+        # the symmetric 6x6 from its 21 upper-triangle entries, in one gather
+        g_cv = ent[:, :, self._voigt_columns()].reshape(T, S, 6, 6)
         gP = ent[:, :, 21:30].reshape(T, S, 3, 3)
         gw = ent[:, :, 30]
         inv, det = _inv3_det(h0)
-        Pt = inv.transpose(-1, -2)
+        Pt = inv.transpose(-1, -2).contiguous()
         w = det / 8.0 if self.model == "corner8" else det
         g_h0 = -(Pt @ gP @ Pt) + (gw * w)[..., None, None] * Pt
         g_free = ent[:, :, 31].clone()
@@ -292,6 +290,17 @@ class DiffField(object):
         g_h0 = g_h0 * mask[..., None, None]
         g_free = g_free * mask
         return g_cv, g_h0, g_free, g_teff
+
+    def _voigt_columns(self):
+        """[36] device index: entry (I, J) of the 6x6 Voigt matrix -> its column among the 21
+        upper-triangle parameter entries (cached)."""
+        cols = self.__dict__.get("_vcols")
+        if cols is None:
+            idx = np.zeros((6, 6), dtype=np.int64)
+            for q, (I, J) in enumerate(voigt_pairs()):
+                idx[I, J] = idx[J, I] = q
+            cols = self._vcols = torch.tensor(idx.ravel(), device=self.device)
+        return cols
 
     def _state_mask(self, nstates):
         """[nrows, max_states] device mask of the states each row has (cached)."""
