@@ -156,7 +156,8 @@ int mm_field_set_params_device(mm_field* f, const int32_t* nstates, const double
  * with the automatic derivatives of micmec/pes/mmff_jax.py:249-250.
  *
  * mm_adjoint_create allocates the work buffers of an object (about (2 * npar + 38) / 32 + 24
- * doubles per work slot) and returns npar = max_states * 32 + 1, the parameter entries per row.
+ * doubles per work slot, and 9 per node) and returns npar = max_states * 32 + 1, the parameter
+ * entries per row.
  *
  * mm_adjoint_compute evaluates, at positions pos [nnodes][3] and domain vectors rvecs [nsys][9]:
  *   sys_out [nsys][38]: energy, virial [9], dE/d(domain vectors) [9] (row-major, vector d at
@@ -178,6 +179,19 @@ int mm_adjoint_create(mm_adjoint** out, mm_field* field, int64_t* npar);
 int mm_adjoint_compute(mm_adjoint* a, const double* pos, const double* dpos, const double* rvecs,
                        const double* drvecs, const double* energy_weight, double* sys_out,
                        double* gpos, double* dgpos, double* params, void* stream);
+
+/* Vector-Jacobian product of (E [nsys], gpos [nnodes][3], vtens [nsys][9]) for the cotangents
+ * gE [nsys] (required; zeros for none), gG [nnodes][3] and gV [nsys][9] (NULL = zero): one
+ * mm_adjoint_compute along the direction (gG + pos gV^T, rvecs gV^T) with energy_weight gE,
+ * framed by two node passes. Outputs: g_pos [nnodes][3] = gE gpos + d(gpos) + gpos gV; sys_out
+ * and params as for mm_adjoint_compute (the position-independent rest of the product is
+ * small: dL/drvecs = gE dE/drvecs + d(dE/drvecs) + (dE/drvecs) gV and dL/dtheta = the sum of
+ * the value and tangent rows of params). node_sys [nnodes] int32 gives each node's system.
+ * Uses work buffers of the object (9 nnodes + 9 nsys doubles, made by mm_adjoint_create). No
+ * reference counterpart (differentiation form). */
+int mm_adjoint_vjp(mm_adjoint* a, const double* pos, const double* rvecs,
+                   const int32_t* node_sys, const double* gE, const double* gG, const double* gV,
+                   double* sys_out, double* g_pos, double* params, void* stream);
 void mm_adjoint_destroy(mm_adjoint* a);
 
 /* ------------------------------------------------------------------------------------------

@@ -57,6 +57,7 @@ _native.register_signature("mm_batch_compute", [_P, _P, _P, _P, _P, _P, _P])
 _native.register_signature("mm_field_set_params", [_P, _P, _P, _P, _P, _P, _f64, _P])
 _native.register_signature("mm_adjoint_create", [ctypes.POINTER(_P), _P, ctypes.POINTER(_i64)])
 _native.register_signature("mm_adjoint_compute", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+_native.register_signature("mm_adjoint_vjp", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 
 NPS = 32       # parameter entries per (type, state) of the adjoint kernel
 SYSQ = 19      # energy, virial (9), domain-vector gradient (9)
@@ -95,17 +96,14 @@ def tensor_from_voigt(cv):
 
 
 def _inv3_det(h):
-    """Inverses and determinants of [..., 3, 3] matrices by cofactors (elementwise tensor
-    arithmetic: no library solve and no synchronisation)."""
-    a, b, c = h[..., 0, 0], h[..., 0, 1], h[..., 0, 2]
-    d, e, f = h[..., 1, 0], h[..., 1, 1], h[..., 1, 2]
-    g, k, l = h[..., 2, 0], h[..., 2, 1], h[..., 2, 2]
-    c00, c01, c02 = e * l - f * k, f * g - d * l, d * k - e * g
-    det = a * c00 + b * c01 + c * c02
-    rows = [torch.stack([c00, c * k - b * l, b * f - c * e], -1),
-            torch.stack([c01, a * l - c * g, c * d - a * f], -1),
-            torch.stack([c02, b * g - a * k, a * e - b * d], -1)]
-    return torch.stack(rows, -2) / det[..., None, None], det
+    """Inverses and determinants of [..., 3, 3] matrices by cofactors: with rows r0, r1, r2 the
+    cofactor rows are r1 x r2, r2 x r0, r0 x r1 and det = r0 . (r1 x r2) (a few batched tensor
+    operations: no library solve and no synchronisation)."""
+    r0, r1, r2 = h[..., 0, :], h[..., 1, :], h[..., 2, :]
+    cof = torch.stack([torch.linalg.cross(r1, r2), torch.linalg.cross(r2, r0),
+                       torch.linalg.cross(r0, r1)], -2)
+    det = (r0 * cof[..., 0, :]).sum(-1)
+    return cof.transpose(-1, -2) / det[..., None, None], det
 
 
 class CellParams(object):
@@ -183,6 +181,7 @@ class DiffField(object):
         counts = np.diff(self.node_start)
         self.node_sys = torch.tensor(np.repeat(np.arange(self.nsys), counts), dtype=torch.long,
                                      device=self.device)
+        self._node_sys32 = self.node_sys.to(torch.int32)
 
     # ------------------------------------------------------------------ inputs
     def positions(self):
@@ -270,6 +269,37 @@ class DiffField(object):
                        dgpos=dgpos, dparams=params[:, :, 1])
         return out
 
+    def vjp(self, pos, rvecs, gE, gG=None, gV=None):
+        """Vector-Jacobian product of (energies [nsys], gpos [N, 3], vtens [nsys, 3, 3]) with the
+        cotangents gE [nsys], gG [N, 3] and gV [nsys, 3, 3] (None: zero), in one library call
+        (``mm_adjoint_vjp``). Returns a dict: ``g_pos`` [N, 3] (the full position cotangent),
+        ``grvecs``/``dgrvecs`` [nsys, 3, 3] (dL/drvecs = gE grvecs + dgrvecs + grvecs gV) and
+        ``params``/``dparams`` [nrows, npar] (raw entries of dL/dtheta = params + dparams)."""
+        kw = dict(dtype=torch.float64, device=self.device)
+        for t in (pos, rvecs, gE, gG, gV):
+            if t is not None and (t.dtype != torch.float64 or not t.is_contiguous()
+                                  or t.device != self.device):
+                raise TypeError("vjp inputs must be contiguous float64 tensors on %s"
+                                % self.device)
+        if pos.shape != (self.N, 3) or rvecs.numel() != 9 * self.nsys or gE.numel() != self.nsys:
+            raise ValueError("pos must be [%d, 3], rvecs [%d, 3, 3] and gE [%d]"
+                             % (self.N, self.nsys, self.nsys))
+        if (gG is not None and gG.shape != pos.shape) or \
+                (gV is not None and gV.numel() != 9 * self.nsys):
+            raise ValueError("gG must be [%d, 3] and gV [%d, 3, 3]" % (self.N, self.nsys))
+        sys_out = torch.empty((self.nsys, 2 * SYSQ), **kw)
+        g_pos = torch.empty((self.N, 3), **kw)
+        params = torch.empty((self.nrows, self.npar, 2), **kw)
+        rc = self.lib.mm_adjoint_vjp(self._adj, _native.ptr(pos), _native.ptr(rvecs),
+                                     _native.ptr(self._node_sys32), _native.ptr(gE),
+                                     _native.ptr(gG), _native.ptr(gV), _native.ptr(sys_out),
+                                     _native.ptr(g_pos), _native.ptr(params),
+                                     _native.stream_ptr())
+        _native.check(rc, "mm_adjoint_vjp")
+        return dict(g_pos=g_pos, grvecs=sys_out[:, 10:19].reshape(-1, 3, 3),
+                    dgrvecs=sys_out[:, SYSQ + 10:].reshape(-1, 3, 3), params=params[:, :, 0],
+                    dparams=params[:, :, 1])
+
     def param_grads(self, entries, params):
         """Map raw kernel entries [nrows, npar] to gradients of (cv, h0, free, teff)."""
         cv, h0, free, teff = [x.detach() for x in params.tensors()]
@@ -348,22 +378,17 @@ class _MicMecEnergy(torch.autograd.Function):
         pos, rv = ctx.saved_tensors
         kw = dict(dtype=torch.float64, device=pos.device)
         gE = torch.zeros(field.nsys, **kw) if gE is None else gE.contiguous()
-        gG = torch.zeros_like(pos) if gG is None else gG.contiguous()
-        gV = (torch.zeros((field.nsys, 3, 3), **kw) if gV is None
-              else gV.contiguous().reshape(-1, 3, 3))
         field.set_params(params)
-        # 3x3 products per node written out elementwise (a batched GEMM of N 1x3x3 products
-        # is far slower): (pos gV^T)[n, l] = sum_k pos[n, k] gV[s(n), l, k]
-        gV_node = gV[field.node_sys]                                   # [N, 3, 3]
-        dpos = (gG + (pos.unsqueeze(1) * gV_node).sum(-1)).contiguous()
-        drv = (rv.unsqueeze(2) * gV.unsqueeze(1)).sum(-1)              # rv gV^T
-        drv = drv.contiguous()
-        out = field.adjoint(pos, rv, dpos, drv, energy_weight=gE)
-        gpos, grv = out["gpos"], out["grvecs"]
-        g_pos = (gE[field.node_sys, None] * gpos + out["dgpos"]
-                 + (gpos.unsqueeze(2) * gV_node).sum(1))
-        g_rv = (gE[:, None, None] * grv + out["dgrvecs"]
-                + (grv.unsqueeze(3) * gV.unsqueeze(1)).sum(2))
+        # one library call: direction (gG + pos gV^T, rvecs gV^T), the dual-number adjoint along
+        # it with energy weights gE, and the position cotangent gE gpos + d(gpos) + gpos gV
+        out = field.vjp(pos, rv, gE, None if gG is None else gG.contiguous(),
+                        None if gV is None else gV.contiguous().reshape(-1, 3, 3))
+        g_pos = out["g_pos"]
+        grv = out["grvecs"]
+        g_rv = gE[:, None, None] * grv + out["dgrvecs"]
+        if gV is not None:
+            gV = gV.contiguous().reshape(-1, 3, 3)
+            g_rv = g_rv + (grv.unsqueeze(3) * gV.unsqueeze(1)).sum(2)
         g_cv, g_h0, g_free, g_teff = field.param_grads(out["params"] + out["dparams"], params)
         return (None, None, g_pos, g_rv.reshape(ctx.rv_shape), g_cv, g_h0, g_free, g_teff)
 

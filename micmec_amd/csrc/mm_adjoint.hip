@@ -428,6 +428,58 @@ segment_sum_kernel(const double* __restrict__ part, int64_t nblk, int nrows_part
   if (threadIdx.x == 0) out[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// Tangent direction of a vector-Jacobian product from the cotangents (gE, gG, gV) of
+// (E, gpos, vtens): dpos_n = gG_n + pos_n gV_s^T (s = node_sys[n]), and for the threads past the
+// nodes drvecs_s = rvecs_s gV_s^T. gG or gV may be null (zero).
+__global__ void __launch_bounds__(256)
+vjp_direction_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
+                     const int32_t* __restrict__ node_sys, const double* __restrict__ gG,
+                     const double* __restrict__ gV, int64_t N, int nsys,
+                     double* __restrict__ dpos, double* __restrict__ drv) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < N) {
+    const double* V = gV ? gV + 9 * (int64_t)node_sys[t] : nullptr;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      double d = gG ? gG[3 * t + l] : 0.0;
+      if (V)
+        d += (pos[3 * t] * V[3 * l] + pos[3 * t + 1] * V[3 * l + 1]) + pos[3 * t + 2] * V[3 * l + 2];
+      dpos[3 * t + l] = d;
+    }
+  } else if (t < N + 9 * (int64_t)nsys) {
+    const int64_t q = t - N;
+    const int64_t sy = q / 9;
+    const int dd = (int)(q % 9) / 3, l = (int)(q % 3);
+    const double* R = rvecs + 9 * sy;
+    double d = 0.0;
+    if (gV) {
+      const double* V = gV + 9 * sy;
+      d = (R[3 * dd] * V[3 * l] + R[3 * dd + 1] * V[3 * l + 1]) + R[3 * dd + 2] * V[3 * l + 2];
+    }
+    drv[q] = d;
+  }
+}
+
+// Position cotangent of the product: g_n = gE_s gpos_n + dgpos_n + gpos_n gV_s (gV may be null:
+// zero).
+__global__ void __launch_bounds__(256)
+vjp_position_kernel(const double* __restrict__ gpos, const double* __restrict__ dgpos,
+                    const int32_t* __restrict__ node_sys, const double* __restrict__ gE,
+                    const double* __restrict__ gV, int64_t N, double* __restrict__ out) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const int sy = node_sys[n];
+  const double e = gE[sy];
+  const double* V = gV ? gV + 9 * (int64_t)sy : nullptr;
+  const double g0 = gpos[3 * n], g1 = gpos[3 * n + 1], g2 = gpos[3 * n + 2];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    double v = e * gpos[3 * n + l] + dgpos[3 * n + l];
+    if (V) v += (g0 * V[l] + g1 * V[3 + l]) + g2 * V[6 + l];
+    out[3 * n + l] = v;
+  }
+}
+
 }  // namespace mm
 
 using namespace mm;
@@ -441,6 +493,7 @@ struct mm_adjoint {
   double* gcell2 = nullptr;    // [8][3][W]
   int32_t* row_ptr = nullptr;  // [nrows + 1] CSR of the segments of every table row
   int32_t* row_idx = nullptr;  // [W / 256]
+  double* vjp = nullptr;       // [3][N][3] + [nsys][9]: dpos, gpos, dgpos, drvecs of mm_adjoint_vjp
 };
 
 extern "C" {
@@ -475,6 +528,8 @@ int mm_adjoint_create(mm_adjoint** out, mm_field* field, int64_t* npar) {
   if (e == hipSuccess) e = hipMalloc((void**)&a->ppart, sizeof(double) * 2 * a->npar * nb);
   if (e == hipSuccess)
     e = hipMalloc((void**)&a->gcell2, sizeof(double) * 24 * (field->W > 0 ? field->W : 1));
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&a->vjp, sizeof(double) * (9 * (size_t)field->N + 9 * field->nsys));
   if (e == hipSuccess) e = hipMalloc((void**)&a->row_ptr, sizeof(int32_t) * row_ptr.size());
   if (e == hipSuccess)
     e = hipMalloc((void**)&a->row_idx, sizeof(int32_t) * (row_idx.empty() ? 1 : row_idx.size()));
@@ -566,6 +621,41 @@ int mm_adjoint_compute(mm_adjoint* a, const double* pos, const double* dpos, con
   return MM_OK;
 }
 
+int mm_adjoint_vjp(mm_adjoint* a, const double* pos, const double* rvecs,
+                   const int32_t* node_sys, const double* gE, const double* gG, const double* gV,
+                   double* sys_out, double* g_pos, double* params, void* stream) {
+  if (!a) return set_error(MM_ERR_ARG, "adjoint is NULL");
+  mm_field* f = a->field;
+  if ((f->N > 0 && (!pos || !node_sys || !g_pos)) || !rvecs || !gE || !sys_out || !params)
+    return set_error(MM_ERR_ARG, "missing argument (gE is required: zeros for none)");
+  double* dpos = a->vjp;
+  double* gtmp = dpos + 3 * f->N;
+  double* dgtmp = gtmp + 3 * f->N;
+  double* drv = dgtmp + 3 * f->N;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(f->device);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nt = f->N + 9 * (int64_t)f->nsys;
+  hipLaunchKernelGGL(vjp_direction_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                     pos, rvecs, node_sys, gG, gV, f->N, f->nsys, dpos, drv);
+  hipError_t e = hipGetLastError();
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess) return hip_error(e, "vector-Jacobian direction kernel");
+  const int rc =
+      mm_adjoint_compute(a, pos, dpos, rvecs, drv, gE, sys_out, gtmp, dgtmp, params, stream);
+  if (rc != MM_OK) return rc;
+  if (f->N > 0) {
+    (void)hipSetDevice(f->device);
+    hipLaunchKernelGGL(vjp_position_kernel, dim3((unsigned)((f->N + 255) / 256)), dim3(256), 0, s,
+                       gtmp, dgtmp, node_sys, gE, gV, f->N, g_pos);
+    e = hipGetLastError();
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_error(e, "vector-Jacobian position kernel");
+  }
+  return MM_OK;
+}
+
 void mm_adjoint_destroy(mm_adjoint* a) {
   if (!a) return;
   if (a->part) (void)hipFree(a->part);
@@ -573,6 +663,7 @@ void mm_adjoint_destroy(mm_adjoint* a) {
   if (a->gcell2) (void)hipFree(a->gcell2);
   if (a->row_ptr) (void)hipFree(a->row_ptr);
   if (a->row_idx) (void)hipFree(a->row_idx);
+  if (a->vjp) (void)hipFree(a->vjp);
   delete a;
 }
 

@@ -377,3 +377,30 @@ def test_device_parameter_table_equals_host_table(setup):
     e3, _, _ = field.compute(pos, rv)
     assert not torch.equal(e3, e2)
     field.set_params(params)
+
+
+def test_vjp_equals_the_adjoint_composition(setup):
+    """mm_adjoint_vjp against the same product composed from mm_adjoint_compute and torch
+    arithmetic: direction (gG + pos gV^T, rvecs gV^T), energy weights gE, position cotangent
+    gE gpos + d(gpos) + gpos gV; missing cotangents are zeros."""
+    field, pos, rv, params = setup
+    field.set_params(params)
+    gen = torch.Generator().manual_seed(9)
+    dev = pos.device
+    gE = torch.rand(field.nsys, generator=gen, dtype=torch.float64).to(dev)
+    gG = torch.randn(tuple(pos.shape), generator=gen, dtype=torch.float64).to(dev)
+    gV = torch.randn((field.nsys, 3, 3), generator=gen, dtype=torch.float64).to(dev)
+    gV_node = gV[field.node_sys]
+    dpos = (gG + (pos.unsqueeze(1) * gV_node).sum(-1)).contiguous()
+    drv = (rv.unsqueeze(2) * gV.unsqueeze(1)).sum(-1).contiguous()
+    ref = field.adjoint(pos, rv, dpos, drv, energy_weight=gE)
+    g_ref = (gE[field.node_sys, None] * ref["gpos"] + ref["dgpos"]
+             + (ref["gpos"].unsqueeze(2) * gV_node).sum(1))
+    out = field.vjp(pos, rv, gE, gG, gV)
+    assert_close_rel(out["g_pos"], g_ref, 1e-13, "g_pos")
+    for key in ("grvecs", "dgrvecs", "params", "dparams"):
+        assert_close_rel(out[key], ref[key], 1e-13, key)
+    zero = field.vjp(pos, rv, gE, torch.zeros_like(gG), torch.zeros_like(gV))
+    none = field.vjp(pos, rv, gE)
+    for key in ("g_pos", "grvecs", "dgrvecs", "params", "dparams"):
+        assert torch.equal(zero[key], none[key]), key

@@ -50,3 +50,5 @@ def test_argument_validation_without_gpu():
                            None) == _native.MM_ERR_ARG
     assert lib.mm_field_set_params_device(None, None, None, None, None, None, 0.0,
                                           None) == _native.MM_ERR_ARG
+    assert lib.mm_adjoint_vjp(None, None, None, None, None, None, None, None, None, None,
+                              None) == _native.MM_ERR_ARG
