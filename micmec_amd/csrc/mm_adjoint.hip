@@ -142,7 +142,7 @@ __device__ __forceinline__ void flush(const double* vsh, int nq, double* out, in
 }
 
 template <typename T, int kModel>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
                const double* __restrict__ rvecs, const double* __restrict__ drvecs,
                const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ cnode,
