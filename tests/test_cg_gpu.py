@@ -154,3 +154,33 @@ def test_engine_convergence_measures_follow_the_reference_formulas():
                                                                        rtol=1e-15)
     # the system holds the accepted point
     assert np.array_equal(np.asarray(s.pos), dof._pos)
+
+
+def test_engine_restarts_along_the_steepest_descent():
+    """A pending direction that is not a descent direction (slope -g.g + beta g.d_old >= 0)
+    makes the device engine restart along -g, as the generic path does."""
+    s, mmf, _ = make("corner8")
+    dof = CartesianDOF(mmf, gpos_rms=1e-30, dpos_rms=1e-30)
+    opt = CGOptimizer(dof)
+    opt.propagate()
+    eng = opt.minimizer.engine
+    e0 = opt.epot
+    eng.set_direction(1e6, abs(eng.gg))   # slope = -g.g + 1e6 |g.g| > 0
+    assert eng.slope() > 0.0
+    opt.propagate()
+    assert opt.minimizer.restarts == 1
+    assert opt.epot < e0
+    # after the restart the search direction was -g of the accepted point before it
+    torch.cuda.synchronize()
+
+
+def test_engine_trial_with_a_nan_position_raises():
+    """A not-a-number energy at a trial point raises ValueError, as the reference's
+    ForcePart.compute does (micmec/pes/mmff.py:135-148)."""
+    s, mmf, _ = make("corner8")
+    dof = CartesianDOF(mmf)
+    opt = CGOptimizer(dof)
+    eng = opt.minimizer.engine
+    eng.x[0] = float("nan")
+    with pytest.raises(ValueError):
+        eng.trial(1e-3)
