@@ -182,6 +182,42 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
   }
 }
 
+// One step of wave_reduce_scatter: the lane keeps the upper H of its 2H values when bit BIT of its
+// lane index is set, the lower H otherwise, and adds its partner's copy of them.
+template <int H, int BIT>
+__device__ __forceinline__ void rs_step(double (&a)[16], int lane) {
+  const bool up = (lane & BIT) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double send = up ? a[i] : a[H + i];
+    const double keep = up ? a[H + i] : a[i];
+    a[i] = keep + __shfl_xor(send, BIT, 64);
+  }
+}
+
+// Wave sums of up to 16 values by a reduce-scatter butterfly (17 shuffles instead of 6 NV):
+// returns in lane l the wave total of value q(l) = 8 b5 + 4 b4 + 2 b3 + b2 (bk = bit k of l);
+// values q >= NV are zero. Fixed pattern, so the sums are deterministic.
+template <int NV>
+__device__ __forceinline__ double wave_reduce_scatter(const double (&v)[NV], int lane) {
+  static_assert(NV <= 16, "at most 16 values");
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) a[q] = q < NV ? v[q] : 0.0;
+  rs_step<8, 32>(a, lane);
+  rs_step<4, 16>(a, lane);
+  rs_step<2, 8>(a, lane);
+  rs_step<1, 4>(a, lane);
+  double t = a[0];
+  t += __shfl_xor(t, 2, 64);
+  t += __shfl_xor(t, 1, 64);
+  return t;
+}
+
+__device__ __forceinline__ int scatter_index(int lane) {
+  return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+}
+
 // Sum SoA block partials part[q * nb + b] over b in a fixed order; one block of 1024 threads,
 // every thread receives the totals.
 template <int NV>

@@ -69,6 +69,13 @@ enum Phase {
 // Block totals of a node pass (one wave per block), written SoA: out[q * gridDim.x + blockIdx.x].
 template <int NV>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* part) {
+  if (NV >= 4) {
+    const int lane = threadIdx.x & 63;
+    const double t = wave_reduce_scatter<NV>(v, lane);
+    const int q = scatter_index(lane);
+    if ((lane & 3) == 0 && q < NV) part[(int64_t)q * gridDim.x + blockIdx.x] = t;
+    return;
+  }
   wave_reduce<NV>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -80,11 +87,11 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* part) {
 template <int NV>
 __device__ __forceinline__ void block_sum_store_wide(double (&v)[NV], double* part) {
   __shared__ double red[NV * (kNodeBlock / 64)];
-  wave_reduce<NV>(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) red[wave * NV + q] = v[q];
+  {
+    const double t = wave_reduce_scatter<NV>(v, lane);
+    const int q = scatter_index(lane);
+    if ((lane & 3) == 0 && q < NV) red[wave * NV + q] = t;
   }
   __syncthreads();
   if (threadIdx.x < NV) {
