@@ -202,7 +202,10 @@ class DiffField(object):
     def set_params(self, params):
         """Make ``params`` (a CellParams) the object's parameter table: a kernel on the current
         stream builds it from the device tensors, so nothing synchronises. Uploading the same,
-        unmodified tensors again (the backward pass after its forward) is skipped."""
+        unmodified tensors again (the backward pass after its forward) is skipped; a change is
+        recognised by the tensors' version counters, which in-place operations on the tensors
+        (or their detached views) advance but writes through ``.data`` do not: after such a
+        write, set ``field._uploaded = None`` to force the upload."""
         ts = params.tensors()
         key = (tuple(ts), tuple(x._version for x in ts), params.nstates.tobytes())
         last = getattr(self, "_uploaded", None)
