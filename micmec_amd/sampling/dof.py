@@ -279,7 +279,7 @@ class CartesianDOF(DOF):
                 not (isinstance(x0, torch.Tensor) and x0.is_cuda):
             return None
         self._cg_engine = _DeviceCGEngine(self, x0)
-        self._host_stage(self._cg_engine.x.view(-1, 3))  # pinned staging, made before the run
+        self._pin_host_arrays()  # made before the run: page-locking 2 x 24 B per node is slow
         return self._cg_engine
 
     def fun(self, x, do_gradient=False):
@@ -324,19 +324,29 @@ class CartesianDOF(DOF):
             else:
                 self._gpos[self.select] = np.asarray(g).reshape(-1, 3)
 
-    def _host_stage(self, like):
-        stage = getattr(self, "_stage", None)
-        if stage is None or stage.shape != like.shape or stage.dtype != like.dtype:
-            stage = self._stage = _pinned_like(like)
-        return stage
+    def _pin_host_arrays(self):
+        """Back the host position and gradient arrays with pinned memory, so that a copy from
+        the device is one DMA transfer straight into them."""
+        if getattr(self, "_pos_pin", None) is None:
+            self._pos_pin = torch.from_numpy(self._pos).pin_memory()
+            self._gpos_pin = torch.from_numpy(self._gpos).pin_memory()
+            self._pos = self._pos_pin.numpy()
+            self._gpos = self._gpos_pin.numpy()
 
     def sync_from_device(self):
         """Copy the last device positions and gradient into the host arrays and the system."""
+        pinned = getattr(self, "_pos_pin", None) is not None
         if getattr(self, "_pos_dev", None) is not None:
-            _to_host(self._pos_dev, self._pos, self._host_stage(self._pos_dev))
+            if pinned and self._pos_dev.shape == self._pos_pin.shape:
+                self._pos_pin.copy_(self._pos_dev.detach())
+            else:
+                self._pos[:] = self._pos_dev.detach().cpu().numpy()
             self.mmf.update_pos(self._pos[:])
         if getattr(self, "_gpos_dev", None) is not None:
-            _to_host(self._gpos_dev, self._gpos, self._host_stage(self._gpos_dev))
+            if pinned and self._gpos_dev.shape == self._gpos_pin.shape:
+                self._gpos_pin.copy_(self._gpos_dev.detach())
+            else:
+                self._gpos[:] = self._gpos_dev.detach().cpu().numpy()
 
     def _device_measures(self):
         """gpos_max, gpos_rms, dpos_max, dpos_rms of the device state (one small copy)."""
