@@ -10,11 +10,13 @@
 //   ncell    [8][N] int32        node n -> 8*w + k of the cell in which n is corner k, -1 none
 //   gcell    [8][3][W]           vertex gradient of every corner of every cell (SoA)
 //   partials [16][W/256]         per-block energy (0) and virial (1..9), SoA
+//   cgpart   [8][(N+255)/256]    per-workgroup sums of the line-search kernels (cg_*)
 // Cells are grouped by type into 256-slot segments, so the type, its number of states and its
 // parameter row are uniform per block and the parameters are read with scalar loads.
 //
 // One lane per cell: the lane gathers the cell's eight nodes, evaluates the corner matrices of
-// every state and accumulates the eight vertex gradients in registers.
+// every state and accumulates the eight vertex gradients in registers (for multistate cells
+// each state's; their weighted sum over the states is kept in LDS).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
