@@ -18,9 +18,10 @@ outputs back-propagates to ``pos``, ``rvecs`` and the parameter tensors
 The energy model is the reference's (micmec/pes/nanocell.py:40-132, nanocell_original.py:46-132,
 micmec/pes/mmff.py:380-398); the reference itself differentiates only with respect to positions
 (mmff.py:384-398) and, in mmff_jax.py:249-250, a deformation of the domain. The forward pass is
-the plain force path (``mm_batch_compute``); the backward pass is one launch of the hand-written
-adjoint kernel in dual numbers (``mm_adjoint_compute``), which recomputes everything from the
-positions: no per-corner intermediate is stored between the passes.
+the plain force path (``mm_batch_compute``); the backward pass is one ``mm_adjoint_vjp`` call:
+the tangent direction below, one launch of the hand-written adjoint kernel in dual numbers
+(``mm_adjoint_compute``), which recomputes everything from the positions (no per-corner
+intermediate is stored between the passes), and the position cotangent.
 
 Backward, for cotangents (gE, gG, gV) of (E, gpos, vtens): with the tangent direction
 u = (dpos, drvecs) = (gG + pos gV^T, rvecs gV^T) the kernel returns the values (gpos, dE/drvecs,
@@ -420,11 +421,16 @@ class FitProblem(object):
         self.s_target = -v / self.volume[:, None, None]
         self.f_scale = float((self.f_target ** 2).mean())
         self.s_scale = float((self.s_target ** 2).mean())
+        # the same squared errors with the signs folded into the targets: mean((g - gt)^2)
+        # and mean((v / V - vt / V)^2), each one fused kernel forward and backward
+        self._g_target = g.contiguous()
+        self._vv_target = (v / self.volume[:, None, None]).contiguous()
+        self._inv_volume = (1.0 / self.volume)[:, None, None].contiguous()
 
     def loss(self, params):
         e, g, v = micmec_energy(self.field, self.pos, self.rvecs, params)
-        lf = ((-g - self.f_target) ** 2).mean() / self.f_scale
-        ls = ((-v / self.volume[:, None, None] - self.s_target) ** 2).mean() / self.s_scale
+        lf = torch.nn.functional.mse_loss(g, self._g_target) / self.f_scale
+        ls = torch.nn.functional.mse_loss(v * self._inv_volume, self._vv_target) / self.s_scale
         return lf + ls
 
 
@@ -460,7 +466,12 @@ def fit(problem, init, steps, lr_cv=1e-2, lr_h0=1e-4, callback=None):
     free, teff = init.free.detach(), init.teff.detach()
     a = torch.zeros_like(cv0, requires_grad=True)
     b = torch.zeros_like(h00, requires_grad=True)
-    opt = torch.optim.Adam([{"params": [a], "lr": lr_cv}, {"params": [b], "lr": lr_h0}])
+    groups = [{"params": [a], "lr": lr_cv}, {"params": [b], "lr": lr_h0}]
+    try:
+        # one fused update kernel per group instead of a dozen foreach launches
+        opt = torch.optim.Adam(groups, fused=True)
+    except (RuntimeError, TypeError, ValueError):
+        opt = torch.optim.Adam(groups)
 
     def params_of(a, b):
         asym = 0.5 * (a + a.transpose(-1, -2))
