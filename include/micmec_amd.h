@@ -329,7 +329,9 @@ int mm_md_init_force(mm_md* md, int with_vtens, void* stream);
 int mm_md_init_properties(mm_md* md, void* stream);
 
 /* Enqueue nsteps integration steps (verlet.py:140-166 with the hooks' pre/post). With
- * use_graph != 0 one step is captured once into a HIP graph and replayed. */
+ * use_graph != 0 one step is captured once into a HIP graph and replayed; the graph is
+ * captured again when new state counts (mm_field_set_params) have changed the field's split
+ * between multistate and single-state cell blocks. */
 int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream);
 
 /* Apply pending velocity/position transforms so that pos and vel hold the reference state. */

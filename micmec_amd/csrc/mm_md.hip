@@ -933,8 +933,9 @@ struct mm_md {
   unsigned int* counters = nullptr;  // [nsys] last-block counters (kept at zero between calls)
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;         // one step
   hipGraphExec_t exec = nullptr;
+  int graph_split = 0;                // the field's block split the graph was captured with
 };
 
 namespace {
@@ -1094,6 +1095,28 @@ int end(mm_md* md, void* stream) {
 
 }  // namespace
 
+namespace {
+
+// Capture one step into an executable graph (replayed once per step).
+hipError_t capture_step(mm_md* md) {
+  hipError_t e = hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t e2 = enqueue_step(md, md->stream);
+  e = hipStreamEndCapture(md->stream, &md->graph);
+  if (e2 != hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0);
+  return e;
+}
+
+void drop_graph(mm_md* md) {
+  if (md->exec) (void)hipGraphExecDestroy(md->exec);
+  if (md->graph) (void)hipGraphDestroy(md->graph);
+  md->exec = nullptr;
+  md->graph = nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 int mm_md_create_batch(mm_md** out, mm_field* field, int32_t nsys, const int64_t* sys_node_start,
@@ -1211,14 +1234,12 @@ int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream) {
   if (rc) return rc;
   hipError_t e = hipSuccess;
   if (use_graph) {
+    // the launch ranges of the cell pass follow the field's block split: a new split (new
+    // state counts through mm_field_set_params) needs a new graph
+    if (md->exec && md->graph_split != md->field->split) drop_graph(md);
     if (!md->exec) {
-      e = hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        hipError_t e2 = enqueue_step(md, md->stream);
-        e = hipStreamEndCapture(md->stream, &md->graph);
-        if (e2 != hipSuccess) e = e2;
-      }
-      if (e == hipSuccess) e = hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0);
+      md->graph_split = md->field->split;
+      e = capture_step(md);
       if (e != hipSuccess) return hip_error(e, "capturing the MD step");
     }
     for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = hipGraphLaunch(md->exec, md->stream);
@@ -1264,8 +1285,7 @@ int mm_md_rmsd(mm_md* md, void* stream) {
 
 void mm_md_destroy(mm_md* md) {
   if (!md) return;
-  if (md->exec) (void)hipGraphExecDestroy(md->exec);
-  if (md->graph) (void)hipGraphDestroy(md->graph);
+  drop_graph(md);
   if (md->stream) {
     (void)hipStreamSynchronize(md->stream);
     (void)hipStreamDestroy(md->stream);

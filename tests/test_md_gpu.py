@@ -119,6 +119,36 @@ def test_graph_and_eager_agree_bitwise():
     assert v1.econs == v2.econs
 
 
+def test_graph_recaptured_when_state_counts_change():
+    """New state counts (mm_field_set_params) move the cell pass's split between the
+    multistate and single-state blocks; the captured step graph is rebuilt, so graph and eager
+    runs still agree bit for bit (a stale graph would skip the blocks that changed kind)."""
+    import ctypes
+    from micmec_amd import _native
+    from micmec_amd.adjoint import voigt_from_tensor
+    from micmec_amd.pes.mmff import parse_type_params
+    runs = []
+    for graph in (True, False):
+        s, v, _ = build((4, 4, 3), 0.3, 0.05, 6, "nve", "corner8")
+        v.use_graph = graph
+        v.run(5)
+        _, ns, h0, c0, free, temp = parse_type_params(s.params, s.types)
+        assert ns.max() == 2
+        cv = np.array([[voigt_from_tensor(c) for c in row] for row in c0])
+        arrays = [np.ascontiguousarray(a, dtype=t) for a, t in
+                  ((np.ones_like(ns), np.int32), (h0, np.float64), (cv, np.float64),
+                   (free, np.float64), (temp, np.float64))]
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        rc = _native.lib().mm_field_set_params(v._engine.fpm.field._handle,
+                                               *[vp(a) for a in arrays], float(units.boltzmann),
+                                               _native.stream_ptr())
+        assert rc == 0
+        v.run(5)
+        runs.append(v)
+    assert np.array_equal(runs[0].pos, runs[1].pos) and np.array_equal(runs[0].vel, runs[1].vel)
+    assert runs[0].epot == runs[1].epot
+
+
 class Recorder(Hook):
     def __init__(self, start, step):
         Hook.__init__(self, start, step)
