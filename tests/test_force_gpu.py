@@ -89,6 +89,27 @@ def test_synthetic_multistate(shape, p_b, p_hole, seed):
         assert rel(v, v0) < TOL, (model, rel(v, v0))
 
 
+@pytest.mark.parametrize("shape,p_b,p_hole,pbc,seed", [
+    ((1, 1, 1), 0.0, 0.0, False, 30),   # one cell, open boundaries
+    ((1, 1, 1), 1.0, 0.0, False, 31),   # one two-state cell
+    ((6, 1, 5), 0.3, 0.1, False, 32),   # a single layer
+    ((2, 3, 7), 0.4, 0.1, True, 33),    # two cells along x under periodicity
+    ((7, 5, 4), 1.0, 0.0, True, 34),    # two-state cells only: no single-state blocks
+    ((9, 9, 9), 0.2, 0.5, True, 35),    # half the cells empty, nodes dropped
+    ((9, 9, 9), 0.2, 0.5, False, 36),
+])
+def test_edge_grids(shape, p_b, p_hole, pbc, seed):
+    print("edge grid", shape, p_b, p_hole, pbc, seed, flush=True)
+    s = make_system(shape, p_b=p_b, p_hole=p_hole, seed=seed, noise=0.1 * units.angstrom,
+                    pbc=pbc)
+    for model in ("corner8", "original"):
+        e, g, v = gpu_eval(s, model)
+        e0, g0, v0 = oracle_eval(s, model)
+        assert rel(e, e0) < TOL, (model, e, e0)
+        assert rel(g, g0) < TOL, (model, rel(g, g0))
+        assert rel(v, v0) < TOL, (model, rel(v, v0))
+
+
 def test_rest_state_is_zero():
     s = make_system((4, 4, 4), seed=1, noise=0.0)
     # At the lattice every corner matrix equals h0, so strains vanish up to the rounding of

@@ -33,8 +33,9 @@ def oracle_force(system, model):
     return force
 
 
-def build(shape, p_b, p_hole, seed, ensemble, model, ext_press=0.0, press=1.0 * units.bar):
-    s = make_system(shape, p_b=p_b, p_hole=p_hole, seed=seed)
+def build(shape, p_b, p_hole, seed, ensemble, model, ext_press=0.0, press=1.0 * units.bar,
+          pbc=True):
+    s = make_system(shape, p_b=p_b, p_hole=p_hole, seed=seed, pbc=pbc)
     pos0 = s.pos.copy()
     rvecs0 = np.array(s.domain.rvecs).copy()
     parts = [ForcePartMechanical(s, model=model)]
@@ -91,6 +92,19 @@ def test_trajectory_parity(ensemble, model):
     if "mtk" in ensemble:
         assert np.abs(verlet.rvecs - omd.rvecs).max() / ANG < 1e-8
         assert np.abs(s.domain.rvecs - omd.rvecs).max() / ANG < 1e-8
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "nvt-nhc"])
+def test_open_boundary_trajectory_parity(ensemble):
+    """A free-standing piece (no domain vectors: ndof 3N - 6, no wrap bits, no pressure)."""
+    print("md open boundaries", ensemble, flush=True)
+    s, verlet, omd = build((4, 3, 3), 0.3, 0.05, 8, ensemble, "corner8", pbc=False)
+    assert s.domain.nvec == 0
+    compare(verlet, omd, "init")
+    for chunk in range(3):
+        verlet.run(25)
+        omd.run(25)
+        compare(verlet, omd, "step %d" % verlet.counter)
 
 
 def test_nve_config0_1000_steps():
