@@ -97,6 +97,8 @@ __device__ __forceinline__ T sum8(T v) {
 }
 
 constexpr int kAdjCells = 32;  // cells per block
+constexpr int kAdjPerSeg = kCellBlock / kAdjCells;  // adjoint blocks per cell-pass block
+static_assert(kCellBlock % kAdjCells == 0, "cell-pass blocks hold whole adjoint blocks");
 constexpr int kNps = 32;       // parameter entries per (row, state)
 constexpr int kSysQ = 19;      // energy, virial (9), domain-vector gradient (9)
 // corner (x + 2y + 4z) -> corner index of the reference order (000,100,010,001,110,101,011,111)
@@ -154,7 +156,7 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
   __shared__ double es_sh[kMaxStates * kAdjCells * 2];
   __shared__ double vsh[kAdjCells * 64];
   const int64_t nblk = gridDim.x;
-  const int seg = blockIdx.x >> 3;
+  const int seg = blockIdx.x / kAdjPerSeg;
   const int trow = blk_row[seg];
   const int sys = blk_sys[seg];
   const int ns = nstates_g[trow];
@@ -419,7 +421,7 @@ segment_sum_kernel(const double* __restrict__ part, int64_t nblk, int nrows_part
     const int seg = list_idx[i];
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s += part[(int64_t)q * nblk + (int64_t)seg * 8 + k];
+    for (int k = 0; k < kAdjPerSeg; ++k) s += part[(int64_t)q * nblk + (int64_t)seg * kAdjPerSeg + k];
     acc += weighted ? weight[blk_sys[seg]] * s : s;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
@@ -492,7 +494,7 @@ struct mm_adjoint {
   double* ppart = nullptr;     // [2 npar][nblk]
   double* gcell2 = nullptr;    // [8][3][W]
   int32_t* row_ptr = nullptr;  // [nrows + 1] CSR of the segments of every table row
-  int32_t* row_idx = nullptr;  // [W / 256]
+  int32_t* row_idx = nullptr;  // [W / kCellBlock]
   double* vjp = nullptr;       // [3][N][3] + [nsys][9]: dpos, gpos, dgpos, drvecs of mm_adjoint_vjp
 };
 
