@@ -601,6 +601,8 @@ int mm_adjoint_compute(mm_adjoint* a, const double* pos, const double* dpos, con
       ForceArgs g{};
       g.ncell = f->ncell;
       g.gcell = f->gcell;
+      g.rstride = f->W;
+      g.rshift = 3;
       g.N = f->N;
       g.W = f->W;
       g.gpos = gpos;

@@ -134,6 +134,8 @@ void free_field(mm_field* f) {
   (void)hipFree(f->cwrap);
   (void)hipFree(f->blk_row);
   (void)hipFree(f->ncell);
+  (void)hipFree(f->nrec);
+  (void)hipFree(f->cbase);
   (void)hipFree(f->nstates);
   (void)hipFree(f->tab);
   (void)hipFree(f->kt);
@@ -178,8 +180,11 @@ void field_args(const mm_field* f, ForceArgs& a) {
   a.max_states = f->max_states;
   a.W = f->W;
   a.N = f->N;
-  a.ncell = f->ncell;
+  a.ncell = f->nrec;
   a.gcell = f->gcell;
+  a.rstride = f->S;
+  a.rshift = 2;
+  a.cbase = f->cbase;
   a.partials = f->partials;
   a.model = f->model;
   a.nanflag = f->nanflag;
@@ -199,6 +204,11 @@ const char* mm_last_error(void) { return g_last_error.c_str(); }
 namespace {
 
 // Shared construction of single-system and batched objects. cell_sys == nullptr means one system.
+// Corner k lies on the upper (+z) face of its cell; its face class (the lower-face corner
+// 0, 1, 2, 4 it coincides with in the +z neighbour, or itself) numbers the gradient planes.
+constexpr int kOffZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+constexpr int kFaceClass[8] = {0, 1, 2, 0, 3, 1, 2, 3};
+
 int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t ncells,
                  int64_t nnodes, const int64_t* surrounding_nodes,
                  const int64_t* surrounding_cells, const uint8_t* wrap_bits,
@@ -241,6 +251,24 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
                          (long long)n, k, (long long)c);
     }
   }
+  // The two tables must describe the same incidence: node n is corner k of cell c exactly when
+  // surrounding_cells[n][k] = c (the cell pass folds the shared faces of neighbouring cells
+  // through surrounding_nodes, the gather finds the sums through surrounding_cells).
+  int64_t listed = 0;
+  for (int64_t n = 0; n < nnodes; ++n) {
+    for (int k = 0; k < 8; ++k) {
+      const int64_t c = surrounding_cells[n * 8 + k];
+      if (c < 0) continue;
+      ++listed;
+      if (surrounding_nodes[c * 8 + k] != n)
+        return set_error(MM_ERR_ARG, "surrounding_cells[%lld][%d] = %lld but corner %d of that "
+                         "cell is node %lld", (long long)n, k, (long long)c, k,
+                         (long long)surrounding_nodes[c * 8 + k]);
+    }
+  }
+  if (listed != 8 * ncells)
+    return set_error(MM_ERR_ARG, "surrounding_cells lists %lld cell corners, the cells have %lld",
+                     (long long)listed, (long long)(8 * ncells));
   std::vector<double> tab, kt;
   int rc = build_param_table(model, nrows, max_states, nstates, h0, c0, free_energy, temp_eff,
                              boltzmann, tab, kt, false);
@@ -296,6 +324,43 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
       if (c >= 0) ncell[(size_t)k * nnodes + n] = (int32_t)(slot[c] * 8 + k);
     }
   }
+  // Shared faces along z, the axis along which the reference numbers consecutive cells: when
+  // slot w + 1 of the same wave holds the +z neighbour of slot w (corner 3 of w is corner 0 of
+  // w + 1), the cell pass adds w's upper-face vertex gradients (corners 3, 5, 6, 7) to w + 1's
+  // lower-face ones (corners 0, 1, 2, 4) with a lane shuffle (cwrap bit 4 on w, bit 3 on
+  // w + 1). Lower faces are stored per slot in four class planes; the upper faces that are not
+  // folded go to compact slots W + i after the work slots, numbered per wave in lane order
+  // from cbase. nrec lists every node's records (slot << 2 | class), skipping folded faces.
+  std::vector<int64_t> cell_of(W, -1);
+  for (int64_t c = 0; c < ncells; ++c) cell_of[slot[c]] = c;
+  for (int64_t w = 0; w + 1 < W; ++w) {
+    const int64_t c = cell_of[w], up = cell_of[w + 1];
+    if (c < 0 || up < 0 || (w + 1) % 64 == 0) continue;
+    if (surrounding_cells[surrounding_nodes[c * 8 + 3] * 8] == up) {
+      cwrap[w] |= 16;
+      cwrap[w + 1] |= 8;
+    }
+  }
+  std::vector<int32_t> cbase(W / 64, 0);
+  std::vector<int64_t> cidx(W, -1);
+  int64_t ncomp = 0;
+  for (int64_t v = 0; v < W / 64; ++v) {
+    cbase[v] = (int32_t)ncomp;
+    for (int64_t w = 64 * v; w < 64 * v + 64; ++w)
+      if (cell_of[w] >= 0 && !(cwrap[w] & 16)) cidx[w] = ncomp++;
+  }
+  const int64_t S = W + ncomp;
+  std::vector<int32_t> nrec((size_t)8 * nnodes, -1);
+  for (int64_t n = 0; n < nnodes; ++n) {
+    for (int k = 0; k < 8; ++k) {
+      const int64_t c = surrounding_cells[n * 8 + k];
+      if (c < 0) continue;
+      const int64_t w = slot[c];
+      const int upper = kOffZ[k], cls = kFaceClass[k];
+      if (!upper) nrec[(size_t)k * nnodes + n] = (int32_t)(4 * w + cls);
+      else if (!(cwrap[w] & 16)) nrec[(size_t)k * nnodes + n] = (int32_t)(4 * (W + cidx[w]) + cls);
+    }
+  }
 
   DeviceGuard guard(device);
   mm_field* f = new mm_field();
@@ -304,6 +369,7 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
   f->C = ncells;
   f->N = nnodes;
   f->W = W;
+  f->S = S;
   f->nrows = nrows;
   f->max_states = max_states;
   f->nsys = nsys;
@@ -311,6 +377,7 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
   const int64_t nb = W / kCellBlock;
   if ((rc = dmalloc(&f->cnode, (size_t)8 * W)) || (rc = dmalloc(&f->cwrap, W)) ||
       (rc = dmalloc(&f->blk_row, nb)) || (rc = dmalloc(&f->ncell, (size_t)8 * nnodes)) ||
+      (rc = dmalloc(&f->nrec, (size_t)8 * nnodes)) || (rc = dmalloc(&f->cbase, W / 64)) ||
       (rc = dmalloc(&f->nstates, nrows)) || (rc = dmalloc(&f->tab, tab.size())) ||
       (rc = dmalloc(&f->kt, kt.size())) || (rc = dmalloc(&f->gcell, (size_t)24 * W)) ||
       (rc = dmalloc(&f->partials, (size_t)kPartialStride * (nb > 0 ? nb : 1))) ||
@@ -324,6 +391,8 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
   if ((rc = h2d(f->cnode, cnode.data(), cnode.size())) || (rc = h2d(f->cwrap, cwrap.data(), W)) ||
       (rc = h2d(f->blk_row, blk_row.data(), blk_row.size())) ||
       (rc = h2d(f->ncell, ncell.data(), ncell.size())) ||
+      (rc = h2d(f->nrec, nrec.data(), nrec.size())) ||
+      (rc = h2d(f->cbase, cbase.data(), cbase.size())) ||
       (rc = h2d(f->nstates, nstates, (size_t)nrows)) ||
       (rc = h2d(f->tab, tab.data(), tab.size())) || (rc = h2d(f->kt, kt.data(), kt.size())) ||
       (rc = h2d(f->blk_sys, blk_sys.data(), blk_sys.size())) ||

@@ -7,8 +7,14 @@
 //   cnode    [W][8] int32        node at corner k of work slot w (-1 = padding slot)
 //   cwrap    [W]    int32        wrap bits of the cell in slot w
 //   blk_row  [W/256] int32       parameter-table row shared by all cells of a block
-//   ncell    [8][N] int32        node n -> 8*w + k of the cell in which n is corner k, -1 none
-//   gcell    [8][3][W]           vertex gradient of every corner of every cell (SoA)
+//   ncell    [8][N] int32        node n -> record (slot << 2 | face class) of its corner k, -1
+//                                none or folded into the +z neighbour's record
+//   gcell    [4][3][S]           vertex gradients by face class: the lower face (corners 0, 1,
+//                                2, 4) of slot w at w, with the upper face of the -z neighbour
+//                                in slot w - 1 added when the two share a wave; the other upper
+//                                faces (corners 3, 5, 6, 7) at compact slots W + i (S = W + their
+//                                number)
+//   cbase    [W/64] int32        first compact slot of every wave
 //   partials [16][W/256]         per-block energy (0) and virial (1..9), SoA
 //   cgpart   [8][(N+255)/256]    per-workgroup sums of the line-search kernels (cg_*)
 // Cells are grouped by type into 256-slot segments, so the type, its number of states and its
@@ -81,6 +87,11 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Corners of the lower (z = 0) and upper (z = 1) face of a cell, in face-class order: upper
+// corner kUpperCorner[q] of a cell is lower corner kLowerCorner[q] of its +z neighbour.
+__device__ constexpr int kLowerCorner[4] = {0, 1, 2, 4};
+__device__ constexpr int kUpperCorner[4] = {3, 5, 6, 7};
+
 // One lane per cell. Energies of all states of the block's cell type, combined by the
 // log-sum-exp of mmff.py:386-398; the vertex gradient of the combined energy goes to gcell and
 // the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials. A block holds
@@ -94,8 +105,8 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             const int32_t* __restrict__ cnode, const int32_t* __restrict__ cwrap,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
-            int32_t max_states, int64_t W, double* __restrict__ gcell,
-            double* __restrict__ partials, int blk0, int nblocks) {
+            int32_t max_states, int64_t W, const int32_t* __restrict__ cbase, int64_t S,
+            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[(kMulti && !kGrad) ? kMaxStates * kCellBlock : 1];
   // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
@@ -120,15 +131,16 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   double acc[10];
 #pragma unroll
   for (int q = 0; q < 10; ++q) acc[q] = 0.0;
-  if (n[0] >= 0) {
-    const int wrap = cwrap[w];
+  double gacc[8][3];
+#pragma unroll
+  for (int v = 0; v < 8; ++v)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gacc[v][j] = 0.0;
+  const bool valid = n[0] >= 0;
+  const int wrap = valid ? cwrap[w] : 0;
+  if (valid) {
     double r[8][3];
     gather_vertices(pos, rv, n, wrap, r);
-    double gacc[8][3];
-#pragma unroll
-    for (int v = 0; v < 8; ++v)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) gacc[v][j] = 0.0;
     double ecell;
     if (!kMulti) {
       ecell = tab[10] + state_energy<kModel, kGrad, true>(tab, r, gacc);
@@ -185,11 +197,6 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
     }
     acc[0] = ecell;
     if (kGrad) {
-#pragma unroll
-      for (int v = 0; v < 8; ++v)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          __builtin_nontemporal_store(gacc[v][j], &gcell[(int64_t)(3 * v + j) * W + w]);
       // virial: vtens[k][l] = sum_v g_v[k] (r_v - r_0)[l]
 #pragma unroll
       for (int v = 1; v < 8; ++v) {
@@ -201,6 +208,38 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
 #pragma unroll
           for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] += gacc[v][k] * d[l];
       }
+    }
+  }
+  if (kGrad) {
+    // The upper face (corners 3, 5, 6, 7) of the slot below is added to this slot's lower face
+    // (corners 0, 1, 2, 4) when that slot holds the -z neighbour (wrap bit 3); the lower faces
+    // go to their class planes, the upper faces that were not folded (wrap bit 4 clear) to the
+    // wave's compact slots, in lane order.
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double t = __shfl_up(gacc[kUpperCorner[q]][j], 1, 64);
+        if (wrap & 8) gacc[kLowerCorner[q]][j] += t;
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          __builtin_nontemporal_store(gacc[kLowerCorner[q]][j], &gcell[(int64_t)(3 * q + j) * S + w]);
+    }
+    const bool open_top = valid && !(wrap & 16);
+    const uint64_t tops = __ballot(open_top);
+    if (open_top) {
+      const int lane = threadIdx.x & 63;
+      const int64_t ci = W + cbase[w >> 6] + __popcll(tops & ((1ull << lane) - 1ull));
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          __builtin_nontemporal_store(gacc[kUpperCorner[q]][j], &gcell[(int64_t)(3 * q + j) * S + ci]);
     }
   }
   block_reduce_store<10, kCellBlock>(acc, red, partials + bid, nblocks);
@@ -249,7 +288,7 @@ __device__ __forceinline__ void block_sum_max_store(double (&v)[NS + NM], double
 template <bool DOT>
 __global__ void __launch_bounds__(256)
 gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell, int64_t N,
-              int64_t W, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag,
+              int64_t RS, int rshift, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag,
               const double* __restrict__ dir, const double* __restrict__ gold,
               double* __restrict__ dpart) {
   __shared__ double sg[768];
@@ -275,11 +314,11 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
     for (int k = 0; k < 8; ++k) {
       const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
       if (e >= 0) {
-        const int64_t w = e >> 3;
-        const int kk = e & 7;
+        const int64_t w = e >> rshift;
+        const int p = e & ((1 << rshift) - 1);
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * kk + j) * W + w]);
+          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
       }
     }
   }
@@ -369,7 +408,8 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
 #define MM_LAUNCH_CELL(MODEL, GRAD, MULTI, S, B0, NBL)                                       \
   hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(NBL), dim3(kCellBlock), 0, S,   \
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
-                     a.tab, a.kt, a.nstates, a.max_states, a.W, a.gcell, a.partials, B0, nb)
+                     a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
+                     a.partials, B0, nb)
 #define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
   do {                                                                                       \
     if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \
@@ -394,7 +434,7 @@ hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
   hipLaunchKernelGGL(gather_kernel<false>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                     a.W, a.gpos, a.accumulate, a.nanflag, nullptr, nullptr, nullptr);
+                     a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -403,7 +443,7 @@ hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, const double
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
   hipLaunchKernelGGL(gather_kernel<true>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                     a.W, a.gpos, a.accumulate, a.nanflag, dir, gold, dpart);
+                     a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag, dir, gold, dpart);
   return hipGetLastError();
 }
 

@@ -28,7 +28,10 @@ struct mm_field {
   int32_t* cnode = nullptr;
   int32_t* cwrap = nullptr;
   int32_t* blk_row = nullptr;
-  int32_t* ncell = nullptr;
+  int32_t* ncell = nullptr;           // [8][N] node -> 8 w + k of its corners (adjoint layout)
+  int32_t* nrec = nullptr;            // [8][N] node -> face-class records of the cell pass
+  int32_t* cbase = nullptr;           // [W/64] see ForceArgs::cbase
+  int64_t S = 0;                      // extent of a gradient plane of the cell pass
   int32_t* nstates = nullptr;
   double* tab = nullptr;
   double* kt = nullptr;

@@ -24,8 +24,11 @@ struct ForceArgs {
   int32_t max_states;
   int64_t W;               // work slots (multiple of kCellBlock)
   int64_t N;
-  const int32_t* ncell;    // [8][N] node -> 8*w + k, -1 = none
-  double* gcell;           // [8][3][W]
+  const int32_t* ncell;    // [8][N] node -> record e (-1 = none): slot e >> rshift, class e & mask
+  double* gcell;           // component j of class p of slot s at (3 p + j) * rstride + s
+  int64_t rstride = 0;     // extent of a gradient plane (slots plus compact upper faces)
+  int rshift = 2;          // 2: four face classes (the cell pass); 3: eight corners (the adjoint)
+  const int32_t* cbase = nullptr;  // [W / 64] first compact upper-face slot of every wave
   double* partials;        // [kPartialStride][W / kCellBlock] (SoA)
   double* energy;          // [1] or null
   double* gpos;            // [N][3] or null

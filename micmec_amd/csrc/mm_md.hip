@@ -233,14 +233,15 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
   if (kMode == kKickDriftLT) block_sum_store<1>(lt, partials + (int64_t)14 * gridDim.x);
 }
 
-// gpos[n] = sum_k gcell[k][:, w(n,k)] (mmff.py:303-318), fused with the second half kick
+// gpos[n] = sum of the node's face-class records (mmff.py:303-318; the cell pass has already
+// added the faces shared along z), fused with the second half kick
 // v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities; with kLT also
 // the Langevin thermostat's post update (the node sums then hold the ekin before it in [13]).
 // One node per thread (the gathers need every wave in flight that they can get).
 template <bool kKick, bool kLT>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
-                   int64_t N, int64_t W, double* __restrict__ gpos, double* __restrict__ v,
+                   int64_t N, int64_t RS, double* __restrict__ gpos, double* __restrict__ v,
                    const double* __restrict__ m, const double* __restrict__ S0,
                    const int32_t* __restrict__ blk_sys, double* __restrict__ partials) {
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
@@ -256,11 +257,11 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     for (int k = 0; k < 8; ++k) {
       const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
       if (e >= 0) {
-        const int64_t w = e >> 3;
-        const int kk = e & 7;
+        const int64_t w = e >> 2;
+        const int p = e & 3;
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * kk + j) * W + w]);
+          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
       }
     }
 #pragma unroll
@@ -967,8 +968,11 @@ ForceArgs force_args(mm_md* md) {
   a.max_states = f->max_states;
   a.W = f->W;
   a.N = f->N;
-  a.ncell = f->ncell;
+  a.ncell = f->nrec;
   a.gcell = f->gcell;
+  a.rstride = f->S;
+  a.rshift = 2;
+  a.cbase = f->cbase;
   a.partials = f->partials;
   a.energy = nullptr;
   a.gpos = nullptr;
@@ -1024,7 +1028,7 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s) {
   if (md->N > 0) {
 #define MM_GATHER(K, T)                                                                      \
   hipLaunchKernelGGL((gather_kick_kernel<K, T>), dim3(md->nbn), dim3(kNodeBlock), 0, s,       \
-                     a.ncell, a.gcell, md->N, a.W, md->gpos, md->vel, md->masses, md->S,      \
+                     a.ncell, a.gcell, md->N, a.rstride, md->gpos, md->vel, md->masses, md->S,      \
                      md->node_blk_sys, md->npart)
     if (kick && lt)
       MM_GATHER(true, true);
