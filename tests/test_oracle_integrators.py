@@ -6,6 +6,9 @@
 * the chain's half-step operator follows the Nose-Hoover-chain equations (high-accuracy ODE);
 * MTK: the conserved quantity converges with the time step, and MTK at pressure P equals MTK at
   pressure 0 plus the external-pressure part P.V (npt.py:645-672);
+* NHC + MTK (TBCombination): the extended energy's drift falls at least 3x per halving of dt
+  over 200 fs at dt = 2, 1, 0.5 fs, and the same check fails once any one of the three coupling
+  terms (G1_add, the chain damping of the barostat velocity, baro_ndof k_B T xi_0) is dropped;
 * seeded initial states follow numpy's legacy stream in the reference's draw order.
 """
 
@@ -132,3 +135,62 @@ def test_random_velocities_follow_legacy_stream():
     t = np.mean(z ** 2 * m.reshape(-1, 1)) / units.boltzmann
     assert np.array_equal(v, z * np.sqrt(300.0 / t))
     assert abs(np.mean(v ** 2 * m.reshape(-1, 1)) / units.boltzmann - 300.0) < 1e-9
+
+
+# ---------------------------------------------------------------- NHC + MTK coupling (TBC)
+#
+# TBCombination couples the chain and the barostat in three places (npt.py:99-148, 653-681,
+# 738-746): the barostat kinetic energy enters the first bead's force (G1_add), the first bead's
+# velocity damps the barostat velocity (exp(-dt v_xi0 / 8) on both sides of every G/4 kick), and
+# the conserved quantity gains baro_ndof k_B T xi_0. With all three the extended energy is a
+# conserved quantity of a second-order splitting and its drift falls with dt; dropping any one
+# of them leaves a drift that no longer converges. The subclasses plant exactly one mistake each.
+
+
+class _DropG1(OracleMD):
+    def _add_press_cont(self):
+        return 0.0
+
+
+class _DropChainDamping(OracleMD):
+    def _update_baro_vel(self, chainvel0, post=0, sub=0):
+        OracleMD._update_baro_vel(self, None, post, sub)
+
+
+class _DropXiTerm(OracleMD):
+    def econs_correction(self):
+        return OracleMD.econs_correction(self) - \
+            self.baro_ndof * (units.boltzmann * self.chain.temp) * self.chain.pos[0]
+
+
+def _tbc_drifts(s, cls, T=200.0, dts=(2.0, 1.0, 0.5)):
+    """Largest |econs(t) - econs(t1)| over T fs of NHC(3) + MTK at each time step."""
+    out = []
+    for dt in dts:
+        np.random.seed(4)
+        md = cls(_force(s), s.pos, np.array(s.domain.rvecs), s.masses, dt * FS,
+                 thermo=dict(temp=300.0, timecon=100 * FS),
+                 baro=dict(temp=300.0, press=units.bar, timecon=1000 * FS))
+        md.propagate()
+        e1, dev = md.econs, 0.0
+        for _ in range(int(round(T / dt)) - 1):
+            md.propagate()
+            dev = max(dev, abs(md.econs - e1))
+        out.append(dev)
+    return out
+
+
+def _converges(drifts):
+    return all(a >= 3.0 * b for a, b in zip(drifts[:-1], drifts[1:]))
+
+
+def test_tbc_nhc_mtk_conserved_quantity_converges(small):
+    d = _tbc_drifts(small, OracleMD)
+    assert _converges(d), d
+
+
+@pytest.mark.parametrize("planted", [_DropG1, _DropChainDamping, _DropXiTerm],
+                         ids=["no-G1_add", "no-chain-damping-of-v_g", "no-baro_ndof-kT-xi0"])
+def test_tbc_convergence_check_catches_dropped_coupling(small, planted):
+    d = _tbc_drifts(small, planted)
+    assert not _converges(d), d
