@@ -65,19 +65,40 @@ class ForcePart(object):
     def __init__(self, name, system):
         self.name = name
         self.energy = 0.0
+        self._stale = set()
         self.gpos = np.zeros((system.nnodes, 3), float)
         self.vtens = np.zeros((3, 3), float)
         self.clear()
 
     def clear(self):
-        """Mark the cached results invalid by filling them with NaN (mmff.py:61-65)."""
+        """Mark the cached results invalid (mmff.py:61-65): the energy becomes NaN at once, the
+        cached ``gpos`` and ``vtens`` read as NaN from then on. The arrays are filled on their
+        next read rather than here, so that invalidating a large system's cache every MD batch
+        costs nothing unless the cache is looked at."""
         self.energy = np.nan
-        if _is_torch(self.gpos):
-            self.gpos.fill_(float("nan"))
-            self.vtens.fill_(float("nan"))
-        else:
-            self.gpos[:] = np.nan
-            self.vtens[:] = np.nan
+        self._stale = {"gpos", "vtens"}
+
+    def _cache(name):
+        key = "_" + name
+
+        def get(self):
+            buf = self.__dict__[key]
+            if name in self._stale:
+                self._stale.discard(name)
+                if _is_torch(buf):
+                    buf.fill_(float("nan"))
+                else:
+                    buf[...] = np.nan
+            return buf
+
+        def put(self, value):
+            self._stale.discard(name)
+            self.__dict__[key] = value
+        return property(get, put)
+
+    gpos = _cache("gpos")
+    vtens = _cache("vtens")
+    del _cache
 
     def update_rvecs(self, rvecs):
         self.clear()
@@ -94,14 +115,12 @@ class ForcePart(object):
         if gpos is None:
             my_gpos = None
         else:
-            self.gpos = _private_like(self.gpos, gpos)
-            my_gpos = self.gpos
+            my_gpos = self.gpos = _private_like(self._gpos, gpos)
             _zero(my_gpos)
         if vtens is None:
             my_vtens = None
         else:
-            self.vtens = _private_like(self.vtens, vtens)
-            my_vtens = self.vtens
+            my_vtens = self.vtens = _private_like(self._vtens, vtens)
             _zero(my_vtens)
         self.energy = self._internal_compute(my_gpos, my_vtens)
         if np.isnan(self.energy):

@@ -1,5 +1,7 @@
 """Base class of iterative algorithms, state items and hooks (micmec/sampling/iterative.py:42-222)."""
 
+import copy
+
 import numpy as np
 
 __all__ = ["Iterative", "StateItem", "AttributeStateItem", "PosStateItem",
@@ -75,97 +77,99 @@ class Iterative(object):
 
 
 class StateItem(object):
+    """One quantity recorded from an iterative algorithm (interface of
+    micmec/sampling/iterative.py:108-222).
+
+    ``update(iterative)`` stores ``value = get_value(iterative)``; the first update also fixes
+    ``shape`` and ``dtype`` (an ndarray's own, otherwise ``()`` and the Python type), which the
+    trajectory writers use to lay out their datasets. ``iter_attrs`` yields ``(name, value)``
+    pairs stored once beside the data. Subclasses provide ``get_value``.
+    """
+
     def __init__(self, key):
         self.key = key
         self.shape = None
         self.dtype = None
 
-    def update(self, iterative):
-        self.value = self.get_value(iterative)
-        if self.shape is None:
-            if isinstance(self.value, np.ndarray):
-                self.shape = self.value.shape
-                self.dtype = self.value.dtype
-            else:
-                self.shape = tuple([])
-                self.dtype = type(self.value)
-
     def get_value(self, iterative):
         raise NotImplementedError
 
+    def update(self, iterative):
+        value = self.get_value(iterative)
+        self.value = value
+        if self.shape is None:
+            self.shape, self.dtype = _layout_of(value)
+
     def iter_attrs(self, iterative):
-        return []
+        return iter(())
 
     def copy(self):
-        return self.__class__()
+        """A fresh item of the same kind and key, its shape not yet known."""
+        twin = copy.copy(self)
+        twin.shape = twin.dtype = None
+        twin.__dict__.pop("value", None)
+        return twin
+
+
+def _layout_of(value):
+    if isinstance(value, np.ndarray):
+        return value.shape, value.dtype
+    return (), type(value)
 
 
 class AttributeStateItem(StateItem):
+    """The iterative's attribute named ``key`` (None where it has none)."""
+
     def get_value(self, iterative):
         return getattr(iterative, self.key, None)
 
-    def copy(self):
-        return self.__class__(self.key)
-
-
-class PosStateItem(StateItem):
-    def __init__(self):
-        StateItem.__init__(self, "pos")
-
-    def get_value(self, iterative):
-        return iterative.mmf.system.pos
-
-
-class TemperatureStateItem(StateItem):
-    def __init__(self):
-        StateItem.__init__(self, "temp")
-
-    def get_value(self, iterative):
-        return getattr(iterative, "temp", None)
-
-    def iter_attrs(self, iterative):
-        yield "ndof", iterative.ndof
-
-
-class VolumeStateItem(StateItem):
-    def __init__(self):
-        StateItem.__init__(self, "volume")
-
-    def get_value(self, iterative):
-        return iterative.mmf.system.domain.volume
-
-
-class DomainStateItem(StateItem):
-    def __init__(self):
-        StateItem.__init__(self, "domain")
-
-    def get_value(self, iterative):
-        return iterative.mmf.system.domain.rvecs
-
 
 class ConsErrStateItem(StateItem):
+    """The attribute ``key`` of the integrator's conserved-energy tracker."""
+
     def get_value(self, iterative):
         return getattr(iterative._cons_err_tracker, self.key, None)
 
-    def copy(self):
-        return self.__class__(self.key)
 
+def _fixed_item(name, key, getter, attrs=None, doc=None):
+    """A StateItem subclass with a fixed key: ``getter(iterative)`` gives the value and
+    ``attrs(iterative)`` the (name, value) attribute pairs."""
 
-class EPotContribStateItem(StateItem):
     def __init__(self):
-        StateItem.__init__(self, "epot_contribs")
+        StateItem.__init__(self, key)
 
-    def get_value(self, iterative):
-        return np.array([part.energy for part in iterative.mmf.parts])
+    body = {"__init__": __init__, "__doc__": doc,
+            "get_value": lambda self, iterative: getter(iterative)}
+    if attrs is not None:
+        body["iter_attrs"] = lambda self, iterative: iter(attrs(iterative))
+    return type(name, (StateItem,), body)
 
-    def iter_attrs(self, iterative):
-        yield "epot_contrib_names", np.array([part.name for part in iterative.mmf.parts],
-                                             dtype="S")
+
+# key, value and attributes of the fixed items (iterative.py:147-192)
+PosStateItem = _fixed_item(
+    "PosStateItem", "pos", lambda it: it.mmf.system.pos,
+    doc="Node positions of the iterative's system.")
+TemperatureStateItem = _fixed_item(
+    "TemperatureStateItem", "temp", lambda it: getattr(it, "temp", None),
+    attrs=lambda it: [("ndof", it.ndof)],
+    doc="Instantaneous temperature; the number of degrees of freedom is stored as attribute.")
+VolumeStateItem = _fixed_item(
+    "VolumeStateItem", "volume", lambda it: it.mmf.system.domain.volume,
+    doc="Volume of the periodic domain.")
+DomainStateItem = _fixed_item(
+    "DomainStateItem", "domain", lambda it: it.mmf.system.domain.rvecs,
+    doc="Domain vectors (rows).")
+EPotContribStateItem = _fixed_item(
+    "EPotContribStateItem", "epot_contribs",
+    lambda it: np.array([part.energy for part in it.mmf.parts]),
+    attrs=lambda it: [("epot_contrib_names",
+                       np.array([part.name for part in it.mmf.parts], dtype="S"))],
+    doc="Energy of every force part; their names are stored as attribute.")
 
 
 class Hook(object):
-    """Base class of routines called during a run every ``step`` iterations from ``start``
-    (iterative.py:200-222)."""
+    """A routine that an iterative algorithm calls at iterations start, start + step,
+    start + 2 step, ... (interface of iterative.py:200-222)."""
     name = None
     kind = None
     method = None
@@ -175,7 +179,8 @@ class Hook(object):
         self.step = step
 
     def expects_call(self, counter):
-        return counter >= self.start and (counter - self.start) % self.step == 0
+        offset = counter - self.start
+        return offset >= 0 and offset % self.step == 0
 
     def __call__(self, iterative):
         raise NotImplementedError

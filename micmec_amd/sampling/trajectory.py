@@ -18,8 +18,18 @@ from .iterative import Hook
 __all__ = ["BaseHDF5Writer", "HDF5Writer", "XYZWriter", "write_xyz_frame"]
 
 
+def _recorded_items(iterative):
+    """The (key, item) pairs a trajectory stores: items that hold a value and whose shape has
+    no empty axis."""
+    for key, item in iterative.state.items():
+        if item.value is None or 0 in tuple(item.shape):
+            continue
+        yield key, item
+
+
 class BaseHDF5Writer(Hook):
-    """Append the iterative's state items to ``f['trajectory']`` (trajectory.py:31-80)."""
+    """Append the iterative's state items, one frame per call, to ``f['trajectory']``
+    (layout of trajectory.py:31-80)."""
 
     def __init__(self, f, start=0, step=1):
         self.f = f
@@ -28,38 +38,38 @@ class BaseHDF5Writer(Hook):
     def __call__(self, iterative):
         if "trajectory" not in self.f:
             self.init_trajectory(iterative)
-        tgrp = self.f["trajectory"]
-        # the row of this frame; a frame left incomplete by an earlier call is overwritten
-        row = min(tgrp[key].shape[0] for key in iterative.state if key in tgrp.keys())
-        for key, item in iterative.state.items():
-            if item.value is None:
-                continue
-            if len(item.shape) > 0 and min(item.shape) == 0:
-                continue
-            ds = tgrp[key]
-            if ds.shape[0] <= row:
-                ds.resize(row + 1, axis=0)
-            ds[row] = item.value
+        group = self.f["trajectory"]
+        frame = self._next_frame(group, iterative)
+        for key, item in _recorded_items(iterative):
+            dataset = group[key]
+            if dataset.shape[0] < frame + 1:
+                dataset.resize(frame + 1, axis=0)
+            dataset[frame] = item.value
+
+    @staticmethod
+    def _next_frame(group, iterative):
+        """Index of the frame to write: the length of the shortest dataset of this iterative's
+        state, so that a frame an interrupted call left incomplete is written again."""
+        lengths = [group[key].shape[0] for key in iterative.state if key in group]
+        return min(lengths)
 
     def dump_system(self, system, grp):
         system.to_hdf5(grp)
 
     def init_trajectory(self, iterative):
-        tgrp = self.f.create_group("trajectory")
-        for key, item in iterative.state.items():
-            if len(item.shape) > 0 and min(item.shape) == 0:
-                continue
-            if item.value is None:
-                continue
-            maxshape = (None,) + item.shape
-            shape = (0,) + item.shape
-            tgrp.create_dataset(key, shape, maxshape=maxshape, dtype=item.dtype)
+        """Create one empty, extensible dataset per recorded item (frames along axis 0) and
+        store the items' attributes on the group."""
+        group = self.f.create_group("trajectory")
+        for key, item in _recorded_items(iterative):
+            group.create_dataset(key, (0,) + tuple(item.shape),
+                                 maxshape=(None,) + tuple(item.shape), dtype=item.dtype)
             for name, value in item.iter_attrs(iterative):
-                tgrp.attrs[name] = value
+                group.attrs[name] = value
 
 
 class HDF5Writer(BaseHDF5Writer):
-    """BaseHDF5Writer that also stores the system once (trajectory.py:83-87)."""
+    """``BaseHDF5Writer`` that stores the system in the file before the first frame
+    (trajectory.py:83-87)."""
 
     def __call__(self, iterative):
         if "system" not in self.f:

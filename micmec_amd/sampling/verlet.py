@@ -326,8 +326,29 @@ class VerletIntegrator(Iterative):
         if getattr(self, "_uploaded", False):
             if self._baro is not None:
                 self.rvecs = self._S[L.S_RVECS:L.S_RVECS + 9].reshape(3, 3).copy()
-                self.mmf.system.domain.update_rvecs(self.rvecs)
+                self.mmf.update_rvecs(self.rvecs)
             self._mirror_hooks()
+
+    def _write_through(self):
+        """Publish the state after a batch of device steps as the reference does after every
+        step (verlet.py:147-151 via mmff.py:191-197): the system's positions follow the
+        integrator (copied from the device on their first read, ``System.defer_pos``), the
+        domain is already passed on by ``_pull``, and the force-field caches are invalidated,
+        keeping the energies of the last evaluation."""
+        pos_t = self._engine.pos
+
+        def fill(arr):
+            arr[...] = pos_t.cpu().numpy().reshape(np.shape(arr))
+        mmf = self.mmf
+        parts = list(getattr(mmf, "parts", []))
+        for part in [mmf] + parts:
+            part.clear()
+        mmf.system.defer_pos(fill)
+        from ..pes.mmff import ForcePartMechanical
+        mmf.energy = self.epot
+        mech = self._s(L.S_EPOT_MECH)
+        for part in parts:
+            part.energy = mech if isinstance(part, ForcePartMechanical) else self.epot - mech
 
     def _s(self, i, n=None):
         if n is None:
@@ -468,6 +489,7 @@ class VerletIntegrator(Iterative):
 
     def propagate(self):
         self._advance(1, True)
+        self._write_through()
         self.call_hooks()
         return False
 
@@ -478,6 +500,7 @@ class VerletIntegrator(Iterative):
             k = self._next_hook_distance(limit)
             due = self._hooks_due(self.counter + k)
             self._advance(k, due)
+            self._write_through()
             if due:
                 self.call_hooks()
             remaining -= k
@@ -496,7 +519,7 @@ class VerletIntegrator(Iterative):
         self._engine.call("mm_md_sync")
         self._engine.call("mm_md_rmsd")
         self._pull()
-        self.mmf.system.pos[:] = self.pos
+        self._write_through()
         self._mirror_hooks()
 
     def _mirror_hooks(self):

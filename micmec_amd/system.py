@@ -117,6 +117,7 @@ class System(object):
 
     def __init__(self, pos, masses, rvecs, surrounding_cells, surrounding_nodes,
                  boundary_nodes=None, grid=None, types=None, params=None):
+        self._pos_source = None
         self.masses = masses
         self.pos = pos
         self.domain = Domain(rvecs)
@@ -128,6 +129,29 @@ class System(object):
         self.boundary_nodes = boundary_nodes
         self.nnodes = len(self.surrounding_cells)
         self.ncells = len(self.surrounding_nodes)
+
+    @property
+    def pos(self):
+        """Node positions [N, 3]. After a device MD run they are fetched from the device on the
+        first read (see ``defer_pos``)."""
+        source = self._pos_source
+        if source is not None:
+            self._pos_source = None
+            source(self._pos)
+        return self._pos
+
+    @pos.setter
+    def pos(self, value):
+        self._pos_source = None
+        self._pos = value
+
+    def defer_pos(self, fill):
+        """Mark the positions as held elsewhere: the next read of ``pos`` first calls
+        ``fill(array)``, which must write the current positions into the array in place.
+        Assigning ``pos`` cancels the pending fill. The device integrator uses this to write its
+        positions through to the system (the reference does so every step, verlet.py:147 via
+        mmff.py:195-197) without a device-to-host copy per run that nobody reads."""
+        self._pos_source = fill
 
     def update_params(self, new_params, type_=1):
         for key, val in new_params.items():

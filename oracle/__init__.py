@@ -16,6 +16,9 @@ nanocell    per-cell energy and gradient of both nanocell models with the refere
             micmec/pes/nanocell_original.py:38-132, micmec/pes/nanocell_utils.py:32-110).
 mmff        ``deformation`` + gpos/vtens/epot assembly (micmec/pes/mmff.py:288-403), as a
             literal per-cell loop and as a chunked vectorised form of the same arithmetic.
+cforce      ctypes front end of ``oracle/c/mm_oracle.c``: the same force-field arithmetic in C
+            with OpenMP over cells (built into ``oracle/libmm_oracle.so``), for grids where the
+            numpy forms are too slow (32^3 and up); pinned against the numpy oracle.
 integrators velocity Verlet with NHC and MTK/TBC hooks (micmec/sampling/verlet.py:71-190,
             nvt.py:361-532, npt.py:52-177, 513-757) and the init helpers of
             micmec/sampling/utils.py.

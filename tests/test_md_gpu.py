@@ -224,3 +224,34 @@ def test_hdf5_writer_records_md_frames():
     assert np.array_equal(traj["domain"][3], verlet.rvecs)
     assert traj["temp"].shape == (4,) and traj["ptens"].shape == (4, 3, 3)
     assert np.array_equal(f["system"]["masses"][:], s.masses)
+
+
+@pytest.mark.parametrize("ensemble", ["nve", "npt-nhc-mtk"])
+def test_positions_written_through_after_run(ensemble):
+    """After ``run(n)`` with no host hook due, the system holds the integrator's positions and
+    domain, and a fresh force evaluation gives the integrator's energy (the reference writes
+    both through on every step: verlet.py:147-151, mmff.py:191-197)."""
+    print("write-through", ensemble, flush=True)
+    s, verlet, omd = build((4, 3, 5), 0.3, 0.1, 31, ensemble, "corner8", ext_press=(
+        50 * units.bar if "mtk" in ensemble else 0.0))
+    mmf = verlet.mmf
+    pos_before = np.array(s.pos, copy=True)
+    verlet.run(7)
+    omd.run(7)
+    assert not np.array_equal(s.pos, pos_before)
+    assert np.array_equal(s.pos, verlet.pos)
+    assert np.abs(s.pos - omd.pos).max() / ANG < 1e-8
+    assert np.array_equal(np.asarray(s.domain.rvecs), verlet.rvecs)
+    # caches: invalidated, energies of the last evaluation kept
+    assert mmf.energy == verlet.epot
+    assert np.isnan(mmf.gpos).all() and np.isnan(mmf.part_micmec.vtens).all()
+    e_parts = sum(p.energy for p in mmf.parts)
+    assert abs(e_parts - verlet.epot) <= 1e-12 * abs(verlet.epot)
+    gpos = np.zeros_like(s.pos)
+    e = mmf.compute(gpos)
+    assert abs(e - verlet.epot) <= 1e-12 * abs(verlet.epot), (e, verlet.epot)
+    assert np.abs(gpos - verlet.gpos).max() <= 1e-12 * np.abs(verlet.gpos).max()
+    # a later run publishes again; an assignment in between is overwritten by it
+    s.pos = s.pos + 1.0
+    verlet.run(3)
+    assert np.array_equal(s.pos, verlet.pos)
