@@ -9,8 +9,9 @@ Cell-steps count grid cells n^3 (empty cells included) times steps. Inputs are s
 resident on the GPU before the timed region starts.
 
 Prints one JSON line. ``roofline`` refers to the dominant kernel (the per-cell force pass),
-timed with HIP events on the stream it is launched on; ``cpu_baseline`` times the numpy oracle
-on a bounded sample of the same workload (a 16^3 grid) on one host core.
+timed with HIP events on the stream it is launched on; ``cpu_baseline`` times the reference's
+algorithm restated as a literal per-cell loop on one host core, on a bounded sample of the same
+workload (an 8^3 grid), with the C restatement on all host cores beside it.
 """
 
 import argparse
@@ -42,8 +43,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
-    ap.add_argument("--cpu-grid", type=int, default=16)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-grid", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-launches", type=int, default=20)
     ap.add_argument("--batched", dest="batched", action="store_true", default=True)
     ap.add_argument("--no-batched", dest="batched", action="store_false")
@@ -131,24 +132,29 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(n, seconds, model):
-    """Numpy oracle on an n^3 sample of the same workload, one host core."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_md(s, force):
     import numpy as np
     from micmec_amd import units
-    from micmec_amd.topology import Grid
-    from micmec_amd.workloads import make_system
-    from oracle import mmff as omm
     from oracle.integrators import OracleMD
-    s = make_system((n, n, n), p_b=0.2, p_hole=0.05, seed=2)
-    wb = Grid(np.rint(s.grid).astype(int)).wrap_bits()
-
-    def force(p, r):
-        return omm.compute_vectorised(p, r, s.types, s.surrounding_nodes, s.surrounding_cells,
-                                      wb, s.params, model)
     np.random.seed(2)
-    md = OracleMD(force, s.pos, np.array(s.domain.rvecs), s.masses, units.femtosecond,
-                  thermo=dict(temp=300.0, timecon=100 * units.femtosecond),
-                  baro=dict(temp=300.0, press=units.bar, timecon=1000 * units.femtosecond))
+    fs = units.femtosecond
+    return OracleMD(force, s.pos, np.array(s.domain.rvecs), s.masses, fs,
+                    thermo=dict(temp=300.0, timecon=100 * fs),
+                    baro=dict(temp=300.0, press=units.bar, timecon=1000 * fs))
+
+
+def _timed_steps(md, seconds):
     steps = 0
     t0 = time.perf_counter()
     while True:
@@ -156,10 +162,58 @@ def cpu_baseline(n, seconds, model):
         steps += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": "numpy oracle (oracle/integrators.py + oracle/mmff.py), %d^3 grid of the "
-                      "same workload, %d NPT steps in %.1f s" % (n, steps, dt)}
+    return steps, time.perf_counter() - t0
+
+
+def cpu_baseline(n, seconds, model):
+    """The reference's CPU path, restated: the literal per-cell loop of micmec/pes/mmff.py:
+    326-403 (one numpy evaluation per cell and state, image shifts from the N x N table of
+    mmff.py:258-286, built once and not timed) driving the Verlet/NHC/MTK sequence, on one host
+    core, on an n^3 sample of the NPT workload for about ``seconds``; beside it the cost per
+    cell evaluation at 3^3..12^3 and the C restatement on all host cores."""
+    import numpy as np
+    from micmec_amd.workloads import make_system
+    from oracle import cforce
+    from oracle import mmff as omm
+    from oracle.topology import get_mic
+
+    def loop_force(s):
+        mic = get_mic(np.rint(s.grid).astype(int), True)
+        d = {"pos": s.pos, "rvecs": s.domain.rvecs, "grid": s.grid, "types": s.types,
+             "surrounding_nodes": s.surrounding_nodes,
+             "surrounding_cells": s.surrounding_cells, "params": s.params}
+
+        def force(p, r):
+            d["pos"], d["rvecs"] = p, r
+            return omm.compute_loop(d, model, mic=mic)
+        return force
+
+    per_cell = {}
+    for m in (3, 4, 6, 8, 12):
+        s = make_system((m, m, m), p_b=0.2, p_hole=0.05, seed=2)
+        f = loop_force(s)
+        t0 = time.perf_counter()
+        f(s.pos, np.array(s.domain.rvecs))
+        per_cell["%d^3" % m] = (time.perf_counter() - t0) / m ** 3 * 1e6
+    s = make_system((n, n, n), p_b=0.2, p_hole=0.05, seed=2)
+    steps, dt = _timed_steps(_oracle_md(s, loop_force(s)), seconds)
+    out = {"value": n ** 3 * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+           "sample": "literal per-cell loop (oracle/mmff.py compute_loop, the reference's "
+                     "mmff.py:258-403 restated) + Verlet/NHC/MTK (oracle/integrators.py), "
+                     "%d^3 grid of the NPT workload, %d steps (3 force evaluations each) in "
+                     "%.1f s" % (n, steps, dt),
+           "us_per_cell_eval": per_cell,
+           "cpu_model": _cpu_model(), "host_cpus": os.cpu_count()}
+    if cforce.available():
+        m = 32
+        s = make_system((m, m, m), p_b=0.2, p_hole=0.05, seed=2)
+        steps, dt = _timed_steps(_oracle_md(s, cforce.make_force(s, model)), 0.5 * seconds)
+        out["all_cores"] = {
+            "value": m ** 3 * steps / dt, "unit": "cell-steps/s", "cores": cforce.num_threads(),
+            "sample": "C restatement of the force field (oracle/c, OpenMP over cells) + numpy "
+                      "Verlet/NHC/MTK, %d^3 grid of the NPT workload, %d steps in %.1f s"
+                      % (m, steps, dt)}
+    return out
 
 
 def batched_vs_sequential(count, steps, seq_steps):
@@ -321,7 +375,7 @@ def main():
     t_cell = kernel_timing(fpm, verlet, args.kernel_launches)
     flops, nbytes, mean_states = algorithmic_counts(system, args.model)
     achieved = flops / t_cell
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+    roofline = {"bound": "fp64_valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": read_traffic(),
                 "kernel": "cell_kernel<corner8>" if args.model == "corner8" else
                           "cell_kernel<original>",

@@ -275,7 +275,16 @@ void mm_adjoint_destroy(mm_adjoint* a);
 #define MM_S_BARO_VOLCON 142  /* 1: volume-preserving cell fluctuations */
 #define MM_S_BARO_EPOT0 143   /* epot and ekin at the start of the barostat call */
 #define MM_S_BARO_EKIN0 144
-#define MM_S_COUNT 160
+/* The MTK barostat's own Nose-Hoover chain (MTKBarostat(baro_thermo=...), npt.py:601-651),
+ * acting on the barostat velocity only; its degrees of freedom are the barostat's. */
+#define MM_S_BCHAIN_ON 145
+#define MM_S_BCHAIN_LEN 146
+#define MM_S_BCHAIN_TEMP 147
+#define MM_S_BCHAIN_NDOF 148
+#define MM_S_BCHAIN_MASS 160 /* 8 */
+#define MM_S_BCHAIN_POS 168  /* 8 */
+#define MM_S_BCHAIN_VEL 176  /* 8 */
+#define MM_S_COUNT 192
 
 /* ------------------------------------------------------------------------------------------
  * Topology of a grid of cells, built on the device: Grid + build_system (micmec/utils.py:55-263)

@@ -283,6 +283,13 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
     groups[(size_t)cell_row[c] * nsys + (cell_sys ? cell_sys[c] : 0)].push_back(c);
   int64_t W = 0;
   for (const auto& g : groups) W += (int64_t)((g.size() + kCellBlock - 1) / kCellBlock) * kCellBlock;
+  // Records are 32-bit: 8 * slot + corner in the per-corner layout, 4 * (W + i) + class in the
+  // face-class layout (i < W), so 8 * W must fit. Padding every (row, system) group to a
+  // whole block can reach that with many small systems long before the cell count limit.
+  if (8 * W > (int64_t)INT32_MAX)
+    return set_error(MM_ERR_ARG, "%lld work slots after padding %lld (row, system) groups to "
+                     "blocks of %d exceed the 32-bit record range; split the batch",
+                     (long long)W, (long long)groups.size(), kCellBlock);
   std::vector<int32_t> cnode((size_t)8 * W, -1), cwrap(W, 0), blk_row(W / kCellBlock, 0),
       blk_sys(W / kCellBlock, 0);
   std::vector<int64_t> slot(ncells, -1);
@@ -333,10 +340,16 @@ int create_field(mm_field** out, int device, int model, int32_t nsys, int64_t nc
   // from cbase. nrec lists every node's records (slot << 2 | class), skipping folded faces.
   std::vector<int64_t> cell_of(W, -1);
   for (int64_t c = 0; c < ncells; ++c) cell_of[slot[c]] = c;
+  // The fold needs the whole face shared: upper corners 3, 5, 6, 7 of w are the nodes of
+  // lower corners 0, 1, 2, 4 of w + 1, pair by pair (the face class numbering).
+  constexpr int kUpper[4] = {3, 5, 6, 7}, kLower[4] = {0, 1, 2, 4};
   for (int64_t w = 0; w + 1 < W; ++w) {
     const int64_t c = cell_of[w], up = cell_of[w + 1];
     if (c < 0 || up < 0 || (w + 1) % 64 == 0) continue;
-    if (surrounding_cells[surrounding_nodes[c * 8 + 3] * 8] == up) {
+    bool shared = true;
+    for (int q = 0; q < 4; ++q)
+      shared = shared && surrounding_nodes[c * 8 + kUpper[q]] == surrounding_nodes[up * 8 + kLower[q]];
+    if (shared) {
       cwrap[w] |= 16;
       cwrap[w + 1] |= 8;
     }

@@ -42,7 +42,6 @@ constexpr int kSliceQ = 10 + kNodeSums;  // force and node sums per slice
 enum NodeMode {
   kKickDrift = 0,
   kScalePos = 1,
-  kScaleVel = 2,
   kVV = 3,
   kApplyV = 4,
   kRmsd = 5,
@@ -55,11 +54,10 @@ enum Phase {
   kPhNveEnd = 2,         // force + node sums after the second kick; properties
   kPhNvtPre = 3,         // NHC pre
   kPhNvtEnd = 4,         // force + node sums; NHC post; properties
-  kPhBaroPre = 5,        // TBC pre, barostat part 1: v_g update, position/domain scaling
-  kPhBaroMid = 6,        // force inside MTK.baro; velocity transform
-  kPhBaroEndPre = 7,     // node sums after the velocity transform; v_g update; NHC pre
-  kPhMainEndNpt = 8,     // force + node sums; NHC post; barostat part 1 of the post call
-  kPhBaroEndPost = 9,    // node sums; v_g update; econs; properties
+  kPhBaroPre = 5,        // TBC pre, barostat: v_g update, position/domain and velocity transforms
+  kPhBaroEndPre = 7,     // force + node sums after the velocity transform; v_g update; NHC pre
+  kPhMainEndNpt = 8,     // force + node sums; NHC post; barostat transforms of the post call
+  kPhBaroEndPost = 9,    // force + node sums; v_g update; econs; properties
   kPhRmsd = 10,          // rmsd_gpos / rmsd_delta
   kPhApplied = 11,       // pending transforms were applied
 };
@@ -207,7 +205,7 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
       vecmat(p, S + MM_S_XMAT, t);
 #pragma unroll
       for (int j = 0; j < 3; ++j) x[3 * n + j] = t[j];
-    } else if (kMode == kScaleVel || kMode == kVV || kMode == kApplyV) {
+    } else if (kMode == kVV || kMode == kApplyV) {
       double vv[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
@@ -227,7 +225,7 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
       }
     }
   }
-  if (kMode == kScaleVel || kMode == kVV || kMode == kRmsd) {
+  if (kMode == kVV || kMode == kRmsd) {
     block_sum_store<13>(acc, partials);
   }
   if (kMode == kKickDriftLT) block_sum_store<1>(lt, partials + (int64_t)14 * gridDim.x);
@@ -237,8 +235,11 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // added the faces shared along z), fused with the second half kick
 // v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities; with kLT also
 // the Langevin thermostat's post update (the node sums then hold the ekin before it in [13]).
+// With kScaleV instead (a barostat call, npt.py:704-735), the pending velocity transforms are
+// applied (the barostat's exp(-(v_g + tr(v_g)/ndof) dt / 2), known before its force) and the
+// node sums of the transformed velocities taken, in the same pass.
 // One node per thread (the gathers need every wave in flight that they can get).
-template <bool kKick, bool kLT>
+template <bool kKick, bool kLT, bool kScaleV = false>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t RS, double* __restrict__ gpos, double* __restrict__ v,
@@ -287,9 +288,18 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
       node_vv(vv, mass, acc);
     }
+    if (kScaleV) {
+      double vv[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
+      apply_vtransform(S, vv);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
+      node_vv(vv, m[n], acc);
+    }
     }
   }
-  if (kKick) {
+  if (kKick || kScaleV) {
     block_sum_store_wide<NA>(acc, partials);
   } else {
     // only the NaN flag: any lane's flag, per block
@@ -372,17 +382,28 @@ __device__ double volume3(const double* r) {
               r[2] * (r[3] * r[7] - r[4] * r[6]));
 }
 
+// Where a Nose-Hoover chain keeps its state in the scalar block: the particle thermostat's
+// chain, or the MTK barostat's own chain (baro_thermo), which acts on the barostat alone.
+struct ChainSlots {
+  int len, temp, ndof, mass, pos, vel;
+};
+constexpr ChainSlots kThermoChain{MM_S_CHAIN_LEN, MM_S_THERMO_TEMP, MM_S_CHAIN_NDOF,
+                                  MM_S_CHAIN_MASS, MM_S_CHAIN_POS, MM_S_CHAIN_VEL};
+constexpr ChainSlots kBaroChain{MM_S_BCHAIN_LEN, MM_S_BCHAIN_TEMP, MM_S_BCHAIN_NDOF,
+                                MM_S_BCHAIN_MASS, MM_S_BCHAIN_POS, MM_S_BCHAIN_VEL};
+
 // NHChain.__call__ (nvt.py:410-451): returns the velocity factor; updates ekin in place.
-__device__ double nhc_chain(double* S, double* ekin_io, bool has_g1, double g1) {
+__device__ double nhc_chain(double* S, double* ekin_io, bool has_g1, double g1,
+                            const ChainSlots& c = kThermoChain) {
 #pragma clang fp contract(off)
-  const int L = (int)S[MM_S_CHAIN_LEN];
+  const int L = (int)S[c.len];
   const double dt = S[MM_S_TIMESTEP];
-  const double temp = S[MM_S_THERMO_TEMP];
+  const double temp = S[c.temp];
   const double kb = S[MM_S_BOLTZMANN];
-  const double ndof = S[MM_S_CHAIN_NDOF];
-  double* mass = S + MM_S_CHAIN_MASS;
-  double* pos = S + MM_S_CHAIN_POS;
-  double* vel = S + MM_S_CHAIN_VEL;
+  const double ndof = S[c.ndof];
+  double* mass = S + c.mass;
+  double* pos = S + c.pos;
+  double* vel = S + c.vel;
   auto bead = [&](int k, double ekin) {
     double g;
     if (k == 0) {
@@ -411,36 +432,52 @@ __device__ double nhc_chain(double* S, double* ekin_io, bool has_g1, double g1) 
 }
 
 // NHChain.get_econs_correction (nvt.py:453-458)
-__device__ double nhc_econs(const double* S) {
+__device__ double nhc_econs(const double* S, const ChainSlots& c = kThermoChain) {
 #pragma clang fp contract(off)
-  const int L = (int)S[MM_S_CHAIN_LEN];
-  const double kt = S[MM_S_BOLTZMANN] * S[MM_S_THERMO_TEMP];
+  const int L = (int)S[c.len];
+  const double kt = S[MM_S_BOLTZMANN] * S[c.temp];
   double kin = 0.0, rest = 0.0;
-  for (int k = 0; k < L; ++k)
-    kin += (S[MM_S_CHAIN_VEL + k] * S[MM_S_CHAIN_VEL + k]) * S[MM_S_CHAIN_MASS + k];
-  for (int k = 1; k < L; ++k) rest += S[MM_S_CHAIN_POS + k];
-  return 0.5 * kin + kt * (S[MM_S_CHAIN_NDOF] * S[MM_S_CHAIN_POS] + rest);
+  for (int k = 0; k < L; ++k) kin += (S[c.vel + k] * S[c.vel + k]) * S[c.mass + k];
+  for (int k = 1; k < L; ++k) rest += S[c.pos + k];
+  return 0.5 * kin + kt * (S[c.ndof] * S[c.pos] + rest);
 }
 
-// MTKBarostat._compute_ekin_baro (npt.py:754-757, anisotropic)
+__device__ __forceinline__ bool baro_chain_on(const double* S) {
+  return S[MM_S_BCHAIN_ON] != 0.0;
+}
+
+// MTKBarostat._compute_ekin_baro (npt.py:748-757): W tr(v_g^T v_g) / 2, or W v_g^2 / 2 in the
+// isotropic form.
 __device__ double baro_ekin(const double* S) {
 #pragma clang fp contract(off)
+  if (S[MM_S_BARO_ANISO] == 0.0)
+    return (0.5 * S[MM_S_MASS_PRESS]) * (S[MM_S_VEL_PRESS] * S[MM_S_VEL_PRESS]);
   double tr = 0.0;
   for (int i = 0; i < 9; ++i) tr += S[MM_S_VEL_PRESS + i] * S[MM_S_VEL_PRESS + i];
   return (0.5 * S[MM_S_MASS_PRESS]) * tr;
 }
 
-// MTKBarostat.add_press_cont (npt.py:738-746)
+// MTKBarostat.add_press_cont (npt.py:738-746); with its own chain the barostat is decoupled
+// from the particle thermostat and contributes 0.
 __device__ double add_press_cont(const double* S) {
 #pragma clang fp contract(off)
+  if (baro_chain_on(S)) return 0.0;
   return 2.0 * baro_ekin(S) - S[MM_S_BARO_NDOF] * (S[MM_S_BARO_TEMP] * S[MM_S_BOLTZMANN]);
+}
+
+// The barostat chain's half step on the barostat kinetic energy (npt.py:625-629, 634-638): the
+// chain advances, the velocity factor it returns is discarded (v_g is damped inside
+// update_baro_vel through chainvel0).
+__device__ void baro_chain_step(double* S) {
+  double ekin = baro_ekin(S);
+  (void)nhc_chain(S, &ekin, true, 0.0, kBaroChain);
 }
 
 __device__ __forceinline__ bool baro_langevin(const double* S) {
   return S[MM_S_BARO] == (double)MM_BARO_LANGEVIN;
 }
 __device__ __forceinline__ bool baro_aniso(const double* S) {
-  return !baro_langevin(S) || S[MM_S_BARO_ANISO] != 0.0;
+  return S[MM_S_BARO_ANISO] != 0.0;
 }
 __device__ __forceinline__ bool thermo_nhc(const double* S) {
   return S[MM_S_THERMO] == (double)MM_THERMO_NHC;
@@ -498,7 +535,17 @@ __device__ void update_baro_vel(double* S, int sys, int post, int sub) {
     }
   }
   if (!lb) {
-    for (int i = 0; i < 9; ++i) vp[i] += (G[i] * dt) / 4.0;
+    // MTK (npt.py:671-676): the isotropic form takes the trace, the volume constraint removes
+    // it (the two together are refused at construction, utils.py:498-500)
+    if (!aniso) {
+      vp[0] += ((((G[0] + G[4]) + G[8])) * dt) / 4.0;
+    } else {
+      if (S[MM_S_BARO_VOLCON] != 0.0) {
+        const double tg = ((G[0] + G[4]) + G[8]) / S[MM_S_NVEC];
+        for (int i = 0; i < 9; i += 4) G[i] -= tg;
+      }
+      for (int i = 0; i < 9; ++i) vp[i] += (G[i] * dt) / 4.0;
+    }
   } else {
     const double W = S[MM_S_MASS_PRESS];
     double R[9];
@@ -552,7 +599,7 @@ __device__ void baro_part2(double* S) {
   if (baro_aniso(S)) {
     double A[9];
     for (int i = 0; i < 9; ++i) A[i] = vp[i];
-    if (!(baro_langevin(S) && S[MM_S_BARO_VOLCON] != 0.0)) {
+    if (S[MM_S_BARO_VOLCON] == 0.0) {
       const double tr = (vp[0] + vp[4]) + vp[8];
       const double sh = tr / S[MM_S_NDOF];
       A[0] += sh;
@@ -607,7 +654,7 @@ __device__ double econs_correction(const double* S) {
   const bool th = S[MM_S_THERMO] != 0.0, ba = S[MM_S_BARO] != 0.0;
   if (th && ba) {
     const double corr = S[MM_S_ECONS_THERMO] + S[MM_S_ECONS_BARO];
-    if (!thermo_nhc(S)) return corr;
+    if (!thermo_nhc(S) || baro_chain_on(S)) return corr;
     // particle NHC thermostat also thermostats the barostat (npt.py:139-148)
     const double kt = S[MM_S_BOLTZMANN] * S[MM_S_THERMO_TEMP];
     return corr + S[MM_S_BARO_NDOF] * kt * S[MM_S_CHAIN_POS];
@@ -712,7 +759,7 @@ __device__ void reduce_blocks(const double* __restrict__ part, int nb,
 
 __device__ __forceinline__ bool phase_uses_force(int phase) {
   return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
-         phase == kPhBaroMid || phase == kPhMainEndNpt;
+         phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost;
 }
 __device__ __forceinline__ bool phase_uses_nodes(int phase) {
   return phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
@@ -769,9 +816,14 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       break;
     }
     case kPhBaroPre:
-      // TBC.pre: chainvel0 = chain.vel[0] before the barostat (npt.py:101-107)
+      // TBC.pre: chainvel0 = chain.vel[0] before the barostat (npt.py:101-107); the
+      // barostat's own chain overrules it (npt.py:616-619)
       S[MM_S_HAS_CHAINVEL0] = nhc ? 1.0 : 0.0;
       S[MM_S_CHAINVEL0] = nhc ? S[MM_S_CHAIN_VEL] : 0.0;
+      if (baro_chain_on(S)) {
+        S[MM_S_HAS_CHAINVEL0] = 1.0;
+        S[MM_S_CHAINVEL0] = S[MM_S_BCHAIN_VEL];
+      }
       // a pending thermostat factor scales m v (x) v by its square
       if (S[MM_S_VSCALE_A_ON] != 0.0) {
         const double fac = S[MM_S_VSCALE_A];
@@ -779,16 +831,17 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       }
       baro_call_begin(S);
       baro_part1(S, sys, 0);
-      break;
-    case kPhBaroMid:
-      take_force(S, f, true);
+      // the velocity transform of this call does not depend on its force: it is applied by
+      // the gather that follows the force
       baro_part2(S);
       break;
     case kPhBaroEndPre: {
       clear_vtransforms(S);
+      take_force(S, f, true);
       take_nodes(S, nd);
       update_baro_vel(S, sys, 0, 1);
       baro_call_end(S);
+      if (baro_chain_on(S)) baro_chain_step(S);
       // a Langevin thermostat's pre update follows in the kick-drift pass
       if (nhc) {
         const double g1 = g1_on ? add_press_cont(S) : 0.0;
@@ -815,21 +868,34 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       } else if (lt) {
         langevin_thermo_econs(S, ekin0, nd);
       }
-      // TBC.post: chainvel0 after the thermostat (npt.py:128-133)
+      // TBC.post: chainvel0 after the thermostat (npt.py:128-133); the barostat's own chain
+      // steps first and overrules it (npt.py:631-639)
       S[MM_S_HAS_CHAINVEL0] = nhc ? 1.0 : 0.0;
       S[MM_S_CHAINVEL0] = nhc ? S[MM_S_CHAIN_VEL] : 0.0;
+      if (baro_chain_on(S)) {
+        baro_chain_step(S);
+        S[MM_S_HAS_CHAINVEL0] = 1.0;
+        S[MM_S_CHAINVEL0] = S[MM_S_BCHAIN_VEL];
+      }
       baro_call_begin(S);
       baro_part1(S, sys, 1);
+      baro_part2(S);
       break;
     }
     case kPhBaroEndPost:
       clear_vtransforms(S);
+      take_force(S, f, true);
       take_nodes(S, nd);
       update_baro_vel(S, sys, 1, 1);
-      if (baro_langevin(S))
+      if (baro_langevin(S)) {
         baro_call_end(S);
-      else
-        S[MM_S_ECONS_BARO] = baro_ekin(S) + S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
+      } else {
+        // MTKBarostat.post (npt.py:640-651)
+        double e = baro_ekin(S);
+        if (S[MM_S_BARO_VOLCON] == 0.0) e += S[MM_S_BARO_PRESS] * S[MM_S_VOLUME];
+        if (baro_chain_on(S)) e += nhc_econs(S, kBaroChain);
+        S[MM_S_ECONS_BARO] = e;
+      }
       properties(S, true);
       break;
     case kPhRmsd: {
@@ -993,7 +1059,6 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
   switch (mode) {
     case kKickDrift: MM_NODE(kKickDrift); break;
     case kScalePos: MM_NODE(kScalePos); break;
-    case kScaleVel: MM_NODE(kScaleVel); break;
     case kVV: MM_NODE(kVV); break;
     case kApplyV: MM_NODE(kApplyV); break;
     case kRmsd: MM_NODE(kRmsd); break;
@@ -1005,8 +1070,8 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
 
 bool reduces(int phase) {
   return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
-         phase == kPhBaroMid || phase == kPhMainEndNpt || phase == kPhInitProps ||
-         phase == kPhBaroEndPre || phase == kPhBaroEndPost || phase == kPhRmsd;
+         phase == kPhMainEndNpt || phase == kPhInitProps || phase == kPhBaroEndPre ||
+         phase == kPhBaroEndPost || phase == kPhRmsd;
 }
 
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
@@ -1020,22 +1085,24 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
 }
 
 // force evaluation: cells, then gather (+ second half kick when kick, + the Langevin
-// thermostat's post update when lt)
-hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s) {
+// thermostat's post update when lt; + the pending velocity transforms when scale_v)
+hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = false) {
   ForceArgs a = force_args(md);
   hipError_t e = launch_cells(a, s);
   if (e != hipSuccess) return e;
   if (md->N > 0) {
-#define MM_GATHER(K, T)                                                                      \
-  hipLaunchKernelGGL((gather_kick_kernel<K, T>), dim3(md->nbn), dim3(kNodeBlock), 0, s,       \
+#define MM_GATHER(K, T, V)                                                                   \
+  hipLaunchKernelGGL((gather_kick_kernel<K, T, V>), dim3(md->nbn), dim3(kNodeBlock), 0, s,    \
                      a.ncell, a.gcell, md->N, a.rstride, md->gpos, md->vel, md->masses, md->S,      \
                      md->node_blk_sys, md->npart)
     if (kick && lt)
-      MM_GATHER(true, true);
+      MM_GATHER(true, true, false);
     else if (kick)
-      MM_GATHER(true, false);
+      MM_GATHER(true, false, false);
+    else if (scale_v)
+      MM_GATHER(false, false, true);
     else
-      MM_GATHER(false, false);
+      MM_GATHER(false, false, false);
 #undef MM_GATHER
     e = hipGetLastError();
   }
@@ -1048,12 +1115,11 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s) {
     if (_e != hipSuccess) return _e;      \
   } while (0)
 
-// The barostat call MTKBarostat.baro after its first scalar phase.
+// The barostat call MTKBarostat.baro after its first scalar phase: positions scaled, force,
+// velocities transformed in the gather, then the second barostat-velocity update.
 hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s) {
   MM_TRY(node(md, kScalePos, s));
-  MM_TRY(force(md, false, false, s));
-  MM_TRY(scalar(md, kPhBaroMid, 1, s));
-  MM_TRY(node(md, kScaleVel, s));
+  MM_TRY(force(md, false, false, s, true));
   MM_TRY(scalar(md, end_phase, 1, s));
   return hipSuccess;
 }

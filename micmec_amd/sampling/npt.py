@@ -1,8 +1,9 @@
 """Barostats and the thermostat-barostat combination (micmec/sampling/npt.py:52-177, 295-787).
 
-``MTKBarostat`` (anisotropic, full 3x3 cell fluctuation, no volume constraint, no barostat
-thermostat), ``LangevinBarostat`` (anisotropic or isotropic, with or without volume constraint)
-and ``TBCombination`` keep the reference constructors. Host-side ``init`` mirrors the reference
+``MTKBarostat`` (anisotropic full 3x3 or isotropic cell fluctuations, with or without volume
+constraint, with or without its own Nose-Hoover chain ``baro_thermo``), ``LangevinBarostat``
+(anisotropic or isotropic, with or without volume constraint) and ``TBCombination`` keep the
+reference constructors. Host-side ``init`` mirrors the reference
 (domain symmetrisation at construction, momentum cleaning, barostat mass, random symmetric
 barostat velocity); the per-step barostat updates, including the extra force evaluation of each
 ``baro`` call and the Langevin barostat's random force, run inside the device integrator.
@@ -62,7 +63,12 @@ class TBCombination(VerletHook):
 
 
 class MTKBarostat(VerletHook):
-    """Martyna-Tobias-Klein barostat with anisotropic cell fluctuations (npt.py:513-757)."""
+    """Martyna-Tobias-Klein barostat (npt.py:513-757): anisotropic (3x3 symmetric barostat
+    velocity) or isotropic (one scalar), optionally volume-preserving, optionally with its own
+    Nose-Hoover chain ``baro_thermo`` (an ``NHCThermostat`` that is not itself a hook of the
+    integrator; it thermostats the barostat alone and decouples it from a particle thermostat).
+    An isotropic barostat with a volume constraint has no degree of freedom and raises
+    ``AssertionError`` as in the reference (utils.py:498-500)."""
     name = "MTTK"
     kind = "deterministic"
     method = "barostat"
@@ -71,10 +77,10 @@ class MTKBarostat(VerletHook):
     def __init__(self, mmf, temp, press, start=0, step=1, timecon=1000 * femtosecond,
                  anisotropic=True, vol_constraint=False, baro_thermo=None, vel_press0=None,
                  restart=False):
-        if not anisotropic or vol_constraint or baro_thermo is not None:
-            raise NotImplementedError("The device MTK barostat supports anisotropic cell "
-                                      "fluctuations without volume constraint or barostat "
-                                      "thermostat.")
+        if baro_thermo is not None and not isinstance(baro_thermo, NHCThermostat):
+            raise TypeError("baro_thermo must be an NHCThermostat.")
+        if baro_thermo is not None and baro_thermo.chain.length > 8:
+            raise ValueError("The barostat thermostat's chain holds at most 8 beads.")
         self.temp = temp
         self.press = press
         self.timecon_press = timecon
@@ -94,6 +100,8 @@ class MTKBarostat(VerletHook):
         VerletHook.__init__(self, start, step)
 
     def init(self, iterative):
+        """MTKBarostat.init (npt.py:580-614), random draws in the reference's order: the
+        barostat velocity, then the barostat chain's velocities."""
         self.timestep_press = iterative.timestep
         if not self.restart:
             clean_momenta(iterative.pos, iterative.vel, iterative.masses,
@@ -105,6 +113,13 @@ class MTKBarostat(VerletHook):
         self.mass_press = (iterative.ndof + self.dim ** 2) * boltzmann * self.temp / angfreq ** 2
         if self.vel_press is None:
             self.vel_press = get_random_vel_press(self.mass_press, self.temp)
+            if not self.anisotropic:
+                self.vel_press = self.vel_press[0][0]
+        if self.baro_thermo is not None:
+            self.baro_thermo.chain.timestep = iterative.timestep
+            self.baro_thermo.chain.set_ndof(self.baro_ndof)
+        if self.vol_constraint:
+            self.vel_press = self.vel_press - np.trace(self.vel_press) / 3 * np.eye(3)
         # gpos and vtens after the symmetrisation (npt.py:611-614)
         iterative._init_force(with_vtens=True)
 
@@ -114,10 +129,14 @@ class MTKBarostat(VerletHook):
     post = pre
 
     def add_press_cont(self):
+        if self.baro_thermo is not None:
+            return 0
         return 2 * self._compute_ekin_baro() - self.baro_ndof * (self.temp * boltzmann)
 
     def _compute_ekin_baro(self):
-        return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
+        if self.anisotropic:
+            return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
+        return 0.5 * self.mass_press * self.vel_press ** 2
 
 
 class LangevinBarostat(VerletHook):
@@ -193,7 +212,9 @@ class MTKAttributeStateItem(StateItem):
         if baro is None:
             raise TypeError("Iterative does not contain an MTKBarostat hook.")
         if self.key.startswith("baro_chain_"):
-            return 0
+            if baro.baro_thermo is None:
+                return 0
+            return getattr(baro.baro_thermo.chain, self.key.split("_")[2])
         return getattr(baro, self.attr)
 
     def copy(self):

@@ -445,6 +445,16 @@ class VerletIntegrator(Iterative):
             vp = np.zeros(9)
             vp[:np.size(baro.vel_press)] = np.asarray(baro.vel_press).ravel()
             S[L.S_VEL_PRESS:L.S_VEL_PRESS + 9] = vp
+            bt = getattr(baro, "baro_thermo", None)
+            if bt is not None:
+                ch = bt.chain
+                S[L.S_BCHAIN_ON] = 1.0
+                S[L.S_BCHAIN_LEN] = ch.length
+                S[L.S_BCHAIN_TEMP] = bt.temp
+                S[L.S_BCHAIN_NDOF] = ch.ndof
+                S[L.S_BCHAIN_MASS:L.S_BCHAIN_MASS + ch.length] = ch.masses
+                S[L.S_BCHAIN_POS:L.S_BCHAIN_POS + ch.length] = ch.pos
+                S[L.S_BCHAIN_VEL:L.S_BCHAIN_VEL + ch.length] = ch.vel
             if lb:
                 S[L.S_LB_TIMECON] = baro.timecon
                 S[L.S_LB_RSCALE] = baro.random_force_scale()
@@ -536,6 +546,10 @@ class VerletIntegrator(Iterative):
             else:
                 self._baro.vel_press = float(S[L.S_VEL_PRESS])
             self._baro.econs_correction = float(S[L.S_ECONS_BARO])
+            bt = getattr(self._baro, "baro_thermo", None)
+            if bt is not None:
+                bt.chain.pos = S[L.S_BCHAIN_POS:L.S_BCHAIN_POS + bt.chain.length].copy()
+                bt.chain.vel = S[L.S_BCHAIN_VEL:L.S_BCHAIN_VEL + bt.chain.length].copy()
         for hook in self.hooks:
             if getattr(hook, "name", None) == "TBCombination":
                 hook.econs_correction = float(S[L.S_ECONS_CORR])

@@ -6,7 +6,9 @@ Follows, in the same order of operations and random draws:
   micmec/sampling/verlet.py:275-307       ConsErrTracker
   micmec/sampling/nvt.py:361-532          NHChain and NHCThermostat
   micmec/sampling/npt.py:52-177           TBCombination
-  micmec/sampling/npt.py:513-757          MTKBarostat (anisotropic, no volume constraint)
+  micmec/sampling/npt.py:513-757          MTKBarostat (anisotropic or isotropic, with or
+                                          without volume constraint, with or without a
+                                          barostat thermostat)
   micmec/sampling/nvt.py:165-218          LangevinThermostat
   micmec/sampling/npt.py:295-510          LangevinBarostat (anisotropic or isotropic, with or
                                           without volume constraint)
@@ -161,9 +163,11 @@ class OracleMD(object):
 
     Parameters mirror the reference constructors: ``thermo = dict(temp, timecon, chainlength)``
     for NHC or ``dict(kind="langevin", temp, timecon, seed)``; ``baro = dict(temp, press,
-    timecon)`` for MTK or ``dict(kind="langevin", temp, press, timecon, anisotropic,
-    vol_constraint, seed)``. ``vel0``/``chain_vel0``/``vel_press0`` skip the corresponding random
-    draws exactly as the reference does. The Langevin noise is the counter-based noise of
+    timecon, anisotropic, vol_constraint, baro_thermo)`` for MTK (``baro_thermo = dict(temp,
+    timecon, chainlength)`` couples a Nose-Hoover chain to the barostat alone) or
+    ``dict(kind="langevin", temp, press, timecon, anisotropic, vol_constraint, seed)``.
+    ``vel0``/``chain_vel0``/``vel_press0`` skip the corresponding random draws exactly as the
+    reference does. The Langevin noise is the counter-based noise of
     ``oracle.philox`` for the given seeds (the reference draws it from numpy's global stream).
     """
 
@@ -190,6 +194,8 @@ class OracleMD(object):
                 pos, rvecs = domain_symmetrize(pos, rvecs)
             # get_ndof_baro (utils.py:478-501)
             self.baro_ndof = (3 * 4 // 2 if self.aniso else 1) - (1 if self.volcon else 0)
+            if self.baro_ndof == 0:
+                raise AssertionError("Isotropic barostat called with a volume constraint.")
         self.pos = pos
         self.rvecs = rvecs
         self.time = 0.0
@@ -222,7 +228,9 @@ class OracleMD(object):
         elif self.thermo_kind == "langevin":
             # LangevinThermostat.init (nvt.py:193-196)
             clean_momenta(self.pos, self.vel, self.masses, self.nvec)
+        self.bchain = None
         if self.baro_kind == "mtk":
+            # MTKBarostat.init (npt.py:580-614)
             clean_momenta(self.pos, self.vel, self.masses, self.nvec)
             if self.ndof is None:
                 self.ndof = get_ndof_internal_md(len(self.pos), self.nvec)
@@ -230,8 +238,21 @@ class OracleMD(object):
             self.mass_press = (self.ndof + 9) * boltzmann * self.baro["temp"] / angfreq ** 2
             if vel_press0 is None:
                 self.vel_press = get_random_vel_press(self.mass_press, self.baro["temp"])
+                if not self.aniso:
+                    self.vel_press = self.vel_press[0][0]
             else:
                 self.vel_press = np.array(vel_press0, dtype=float)
+            bt = self.baro.get("baro_thermo")
+            if bt is not None:
+                self.bchain = NHChain(bt.get("chainlength", 3), timestep, bt["temp"],
+                                      bt.get("timecon", 100 * femtosecond))
+                if bt.get("chain_vel0") is not None:
+                    self.bchain.set_ndof(self.baro_ndof, draw=False)
+                    self.bchain.vel = np.array(bt["chain_vel0"], dtype=float)
+                else:
+                    self.bchain.set_ndof(self.baro_ndof)
+            if self.volcon:
+                self.vel_press -= np.trace(self.vel_press) / 3 * np.eye(3)
             self.epot, self.gpos, self.vtens = self._force(self.pos, self.rvecs)
         elif self.baro_kind == "langevin":
             # LangevinBarostat.init (npt.py:351-372)
@@ -284,7 +305,9 @@ class OracleMD(object):
 
     # ------------------------------------------------------------ MTK (npt.py:616-757)
     def _baro_ekin(self):
-        return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
+        if self.aniso:
+            return 0.5 * self.mass_press * np.trace(np.dot(self.vel_press.T, self.vel_press))
+        return 0.5 * self.mass_press * self.vel_press ** 2
 
     def _update_baro_vel(self, chainvel0, post=0, sub=0):
         dt = self.timestep
@@ -300,6 +323,12 @@ class OracleMD(object):
                           - self.baro["press"] * volume_of(self.rvecs)) * np.eye(3)) \
             / self.mass_press
         if not lb:
+            # MTK (npt.py:671-676): the isotropic form keeps the trace; the volume constraint
+            # removes it (the two together are refused at construction)
+            if not self.aniso:
+                G = np.trace(G)
+            if self.volcon:
+                G -= np.trace(G) / self.nvec * np.eye(self.nvec)
             self.vel_press += G * dt / 4
         else:
             # LangevinBarostat.getR (npt.py:489-505) with the counter-based draw
@@ -335,7 +364,7 @@ class OracleMD(object):
             self.rvecs = c * self.rvecs
         self.epot, self.gpos, self.vtens = self._force(self.pos, self.rvecs)
         if self.aniso:
-            if self.baro_kind == "langevin" and self.volcon:
+            if self.volcon:
                 Dg, Eg = np.linalg.eigh(self.vel_press)
             else:
                 Dg, Eg = np.linalg.eigh(self.vel_press + (np.trace(self.vel_press) / self.ndof)
@@ -350,7 +379,17 @@ class OracleMD(object):
             self.econs_baro += epot0 - self.epot + ekin0 - self.ekin
 
     def _add_press_cont(self):
+        # MTKBarostat.add_press_cont (npt.py:738-746): a barostat thermostat decouples them
+        if self.bchain is not None:
+            return 0
         return 2 * self._baro_ekin() - self.baro_ndof * (self.baro["temp"] * boltzmann)
+
+    def _bchain_call(self):
+        """The barostat thermostat's half step on the barostat kinetic energy (npt.py:625-629,
+        634-638); the barostat velocity itself is scaled inside update_baro_vel."""
+        vp = np.zeros(np.shape(self.vel_press))
+        vp[...] = self.vel_press
+        self.bchain(self._baro_ekin(), vp, 0)
 
     # ------------------------------------------------------------ hooks in TBC order
     def _thermo_call(self, post):
@@ -366,22 +405,34 @@ class OracleMD(object):
     def _pre(self):
         if self.baro is not None:
             chainvel0 = self.chain.vel[0] if self.chain is not None else None
+            if self.bchain is not None:
+                chainvel0 = self.bchain.vel[0]
             self._baro(chainvel0, 0)
+            if self.bchain is not None:
+                self._bchain_call()
         self._thermo_call(0)
 
     def _post(self):
         self._thermo_call(1)
         if self.baro is not None:
             chainvel0 = self.chain.vel[0] if self.chain is not None else None
+            if self.bchain is not None:
+                self._bchain_call()
+                chainvel0 = self.bchain.vel[0]
             self._baro(chainvel0, 1)
             if self.baro_kind == "mtk":
-                self.econs_baro = self._baro_ekin() + self.baro["press"] * volume_of(self.rvecs)
+                # MTKBarostat.post (npt.py:640-651)
+                self.econs_baro = self._baro_ekin()
+                if not self.volcon:
+                    self.econs_baro += self.baro["press"] * volume_of(self.rvecs)
+                if self.bchain is not None:
+                    self.econs_baro += self.bchain.econs_correction()
 
     def econs_correction(self):
         corr = 0.0
         if self.thermo is not None and self.baro is not None:
             corr = self.econs_thermo + self.econs_baro
-            if self.chain is not None:
+            if self.chain is not None and self.bchain is None:
                 corr += self.baro_ndof * (boltzmann * self.chain.temp) * self.chain.pos[0]
         elif self.thermo is not None:
             corr = self.econs_thermo

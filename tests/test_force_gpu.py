@@ -200,3 +200,47 @@ def test_pressure_part():
     assert abs(e - (e0 + p * V)) <= 1e-14 * abs(e)
     assert rel(v, v0 + p * V * np.identity(3)) < 1e-14
     assert np.array_equal(g, g0)
+
+
+def test_partial_face_contact_is_not_folded():
+    """Two consecutive cells that share only one node (corner 3 of the first is corner 0 of the
+    second, the other upper/lower corners differ) must not have their faces folded together:
+    the device's gradient of every node equals the oracle's."""
+    from micmec_amd.pes.mmff import DeviceField
+    from micmec_amd.workloads import type_a
+    from oracle import cforce
+    print("partial face contact", flush=True)
+    sn = np.array([[0, 1, 2, 3, 4, 5, 6, 7],
+                   [3, 8, 9, 10, 11, 12, 13, 14]], dtype=np.int64)
+    sc = -np.ones((15, 8), dtype=np.int64)
+    for c in range(2):
+        for k in range(8):
+            sc[sn[c, k], k] = c
+    offs = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1),
+                     (0, 1, 1), (1, 1, 1)], float)
+    a = 20.7 * units.angstrom
+    rng = np.random.default_rng(3)
+    pos = np.zeros((15, 3))
+    pos[sn[0]] = offs * a
+    pos[sn[1]] = pos[3] + offs * a + np.array([0.3, -0.2, 0.1]) * a
+    pos += rng.normal(0, 0.05 * units.angstrom, pos.shape)
+    t = type_a()
+    params = {"type1/" + k: v for k, v in t.items()}
+    types = np.array([1, 1])
+    tab = cforce.ParamTable(params, types)
+    e0, g0, v0 = cforce.compute_c(pos, np.zeros((0, 3)), types, sn, sc, np.zeros(2, np.uint8),
+                                  params, "corner8")
+    for model in ("corner8",):
+        f = DeviceField(sn, sc, np.zeros(2, np.uint8), tab.cell_row, tab.nstates,
+                        tab.h0.reshape(-1, tab.smax, 3, 3), tab.c0.reshape(-1, tab.smax, 3, 3, 3, 3),
+                        tab.free, tab.temp, model=model)
+        kw = dict(dtype=torch.float64, device=f.device)
+        P = torch.tensor(pos, **kw)
+        E = torch.zeros(1, **kw)
+        G = torch.zeros((15, 3), **kw)
+        V = torch.zeros((3, 3), **kw)
+        f.compute(P, torch.zeros((3, 3), **kw), E, G, V)
+        torch.cuda.synchronize()
+        assert rel(E.item(), e0) < TOL
+        assert rel(G.cpu().numpy(), g0) < TOL
+        assert rel(V.cpu().numpy(), v0) < TOL

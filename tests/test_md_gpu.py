@@ -255,3 +255,45 @@ def test_positions_written_through_after_run(ensemble):
     s.pos = s.pos + 1.0
     verlet.run(3)
     assert np.array_equal(s.pos, verlet.pos)
+
+
+@pytest.mark.parametrize("variant", ["isotropic", "vol_constraint", "baro_thermo",
+                                     "isotropic-nph", "baro_thermo-nph"])
+def test_mtk_variants_trajectory_parity(variant):
+    """MTKBarostat(anisotropic=False / vol_constraint=True / baro_thermo=NHCThermostat(...)),
+    with or without a particle NHC thermostat: 100 steps against the oracle (npt.py:518-757)."""
+    print("mtk variant", variant, flush=True)
+    s = make_system((4, 3, 5), p_b=0.3, p_hole=0.1, seed=37)
+    pos0, rvecs0 = s.pos.copy(), np.array(s.domain.rvecs).copy()
+    mmf = MicMecForceField(s, [ForcePartMechanical(s)])
+    kw, okw = {}, {}
+    if variant.startswith("isotropic"):
+        kw["anisotropic"] = okw["anisotropic"] = False
+    if variant.startswith("vol_constraint"):
+        kw["vol_constraint"] = okw["vol_constraint"] = True
+    if variant.startswith("baro_thermo"):
+        kw["baro_thermo"] = NHCThermostat(temp=300.0, timecon=500 * FS)
+        okw["baro_thermo"] = dict(temp=300.0, timecon=500 * FS, chainlength=3)
+    nph = variant.endswith("-nph")
+    np.random.seed(37)
+    hooks = [] if nph else [NHCThermostat(temp=300.0, timecon=100 * FS)]
+    baro = MTKBarostat(mmf, temp=300.0, press=100 * units.bar, timecon=1000 * FS, **kw)
+    hooks.append(baro)
+    verlet = VerletIntegrator(mmf, timestep=1.0 * FS, hooks=hooks, temp0=300.0)
+    np.random.seed(37)
+    omd = OracleMD(oracle_force(s, "corner8"), pos0, rvecs0, s.masses, 1.0 * FS, temp0=300.0,
+                   thermo=None if nph else dict(temp=300.0, timecon=100 * FS, chainlength=3),
+                   baro=dict(temp=300.0, press=100 * units.bar, timecon=1000 * FS, **okw))
+    compare(verlet, omd, variant + " init")
+    verlet.run(100)
+    omd.run(100)
+    compare(verlet, omd, variant + " 100 steps")
+    assert np.abs(verlet.rvecs - omd.rvecs).max() <= 1e-12 * np.abs(omd.rvecs).max()
+    if variant.startswith("baro_thermo"):
+        assert np.abs(baro.baro_thermo.chain.vel - omd.bchain.vel).max() <= \
+            1e-9 * np.abs(omd.bchain.vel).max()
+    if variant.startswith("vol_constraint"):
+        assert abs(verlet.mmf.system.domain.volume / abs(np.linalg.det(rvecs0)) - 1.0) < 1e-12
+    if variant.startswith("isotropic"):
+        assert np.ndim(baro.vel_press) == 0
+        assert abs(baro.vel_press - omd.vel_press) <= 1e-9 * abs(omd.vel_press)

@@ -194,3 +194,102 @@ def test_tbc_nhc_mtk_conserved_quantity_converges(small):
 def test_tbc_convergence_check_catches_dropped_coupling(small, planted):
     d = _tbc_drifts(small, planted)
     assert not _converges(d), d
+
+
+# ---------------------------------------------------------------- MTK variants (npt.py:518-757)
+
+
+def _mtk_drifts(s, cls, baro, thermo, T, dts):
+    out, runs = [], []
+    for dt in dts:
+        np.random.seed(4)
+        md = cls(_force(s), s.pos, np.array(s.domain.rvecs), s.masses, dt * FS,
+                 thermo=dict(temp=300.0, timecon=100 * FS) if thermo else None,
+                 baro=dict(temp=300.0, press=units.bar, timecon=1000 * FS, **baro))
+        md.propagate()
+        e1, dev = md.econs, 0.0
+        vols = []
+        for _ in range(int(round(T / dt)) - 1):
+            md.propagate()
+            dev = max(dev, abs(md.econs - e1))
+            vols.append(abs(np.linalg.det(md.rvecs)))
+        out.append(dev)
+        runs.append(np.array(vols))
+    return out, runs
+
+
+class _IsoWithoutNdofTerm(OracleMD):
+    """Isotropic velocity factor exp(-v_g dt / 2) instead of exp(-(1 + 3 / ndof) v_g dt / 2)."""
+
+    def _baro(self, chainvel0, post=0):
+        dt = self.timestep
+        self._update_baro_vel(chainvel0, post, 0)
+        c = np.exp(self.vel_press * dt / 2)
+        self.pos, self.rvecs = c * self.pos, c * self.rvecs
+        self.epot, self.gpos, self.vtens = self._force(self.pos, self.rvecs)
+        self.vel = np.exp(-self.vel_press * dt / 2) * self.vel
+        self.ekin = self.ekin_now()
+        self._update_baro_vel(chainvel0, post, 1)
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    # ndof = 21: the 3 / ndof term of the isotropic velocity factor is 14%
+    return make_system((2, 2, 2), seed=5, noise=0.2 * units.angstrom)
+
+
+def test_mtk_isotropic_converges(tiny):
+    # the isotropic volume mode of this material is stiff (period ~2.5 fs), hence dt <= 1 fs
+    d, _ = _mtk_drifts(tiny, OracleMD, dict(anisotropic=False), False, 800.0, (1.0, 0.5, 0.25))
+    assert _converges(d), d
+    bad, _ = _mtk_drifts(tiny, _IsoWithoutNdofTerm, dict(anisotropic=False), False, 800.0,
+                         (1.0, 0.5, 0.25))
+    assert not _converges(bad), bad
+
+
+class _VolconUnprojected(OracleMD):
+    def _update_baro_vel(self, chainvel0, post=0, sub=0):
+        self.volcon = False
+        try:
+            OracleMD._update_baro_vel(self, chainvel0, post, sub)
+        finally:
+            self.volcon = True
+
+
+def test_mtk_volume_constraint(small):
+    d, vols = _mtk_drifts(small, OracleMD, dict(vol_constraint=True), True, 200.0,
+                          (1.0, 0.5, 0.25))
+    # second order overall (the first halving is not yet asymptotic: 2.9 then 3.7)
+    assert d[0] >= 8.0 * d[2], d
+    for v in vols:
+        assert np.abs(v / v[0] - 1.0).max() < 1e-12
+    _, vols_bad = _mtk_drifts(small, _VolconUnprojected, dict(vol_constraint=True), True, 50.0,
+                              (1.0,))
+    assert np.abs(vols_bad[0] / vols_bad[0][0] - 1.0).max() > 1e-6
+
+
+def test_mtk_isotropic_volume_constraint_refused(small):
+    with pytest.raises(AssertionError):
+        OracleMD(_force(small), small.pos, np.array(small.domain.rvecs), small.masses, FS,
+                 baro=dict(temp=300.0, press=units.bar, anisotropic=False,
+                           vol_constraint=True))
+
+
+class _BaroChainWithoutEcons(OracleMD):
+    def _post(self):
+        OracleMD._post(self)
+        self.econs_baro -= self.bchain.econs_correction()
+
+
+class _BaroChainStillCoupled(OracleMD):
+    def _add_press_cont(self):
+        return 2 * self._baro_ekin() - self.baro_ndof * (self.baro["temp"] * units.boltzmann)
+
+
+@pytest.mark.parametrize("cls,ok", [(OracleMD, True), (_BaroChainWithoutEcons, False),
+                                    (_BaroChainStillCoupled, False)],
+                         ids=["reference", "no-chain-econs", "G1_add-kept"])
+def test_mtk_barostat_thermostat(small, cls, ok):
+    bt = dict(baro_thermo=dict(temp=300.0, timecon=500 * FS))
+    d, _ = _mtk_drifts(small, cls, bt, True, 200.0, (2.0, 1.0, 0.5))
+    assert _converges(d) == ok, d
