@@ -54,6 +54,12 @@ __device__ constexpr int kHi[8][3] = {
 // Corner offsets along x, y, z.
 __device__ constexpr int kOff[8][3] = {
     {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+// The twelve edges: four along x, y and z each; edge kEdge[a][d] is the edge along d through
+// corner a, from corner kEdgeLo to corner kEdgeHi (so kEdgeLo/Hi[kEdge[a][d]] = kLo/kHi[a][d]).
+__device__ constexpr int kEdgeLo[12] = {0, 2, 3, 6, 0, 1, 3, 5, 0, 1, 2, 4};
+__device__ constexpr int kEdgeHi[12] = {1, 4, 5, 7, 2, 4, 6, 7, 3, 5, 6, 7};
+__device__ constexpr int kEdge[8][3] = {{0, 4, 8},  {0, 5, 9},  {1, 4, 10}, {2, 6, 8},
+                                        {1, 5, 11}, {2, 7, 9},  {3, 6, 10}, {3, 7, 11}};
 
 // Strain energy density of one corner (or of the averaged matrix) and, if kGrad, the
 // derivative with respect to the corner matrix M, both for one (type, state) entry `tab`.
@@ -118,14 +124,20 @@ template <int kModel, bool kGrad, bool kUnit = false>
 __device__ __forceinline__ double state_energy(const double* __restrict__ tab, const double r[8][3],
                                                double g[8][3], double gscale = 1.0) {
   if (kModel == 0) {
-    double usum = 0.0;
+    // the twelve edges once (each is a row of two corner matrices)
+    double E[12][3];
+#pragma unroll
+    for (int e = 0; e < 12; ++e)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) E[e][j] = r[kEdgeHi[e]][j] - r[kEdgeLo[e]][j];
+    double usum = -0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
       double M[9], G[9];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
 #pragma unroll
-        for (int j = 0; j < 3; ++j) M[3 * d + j] = r[kHi[a][d]][j] - r[kLo[a][d]][j];
+        for (int j = 0; j < 3; ++j) M[3 * d + j] = E[kEdge[a][d]][j];
       }
       usum += corner_energy<kGrad>(tab, M, G, kCvwOffset);
       if (kGrad) {

@@ -52,27 +52,41 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* smem
   }
 }
 
-// Gather the eight vertices of a cell: r_0 = pos[n_0], r_k = r_0 + ((pos[n_k] - pos[n_0])
-// + sum_d shift_kd rvecs[d]) with shift_kd = 1 when the cell is in the last layer along a
-// periodic axis d and corner k lies at +d (mmff.py:347-371 with get_mic 258-286).
+// Gather the eight vertices of a cell relative to corner 0: r_0 = 0, r_k = (pos[n_k] -
+// pos[n_0]) + sum_d shift_kd rvecs[d] with shift_kd = 1 when the cell is in the last layer
+// along a periodic axis d and corner k lies at +d (mmff.py:347-371 with get_mic 258-286). The
+// reference adds pos[n_0] back before taking edge differences; the energy depends on the edges
+// only and the virial on r_v - r_0, and leaving the origin out avoids rounding the vertices at
+// the magnitude of the absolute coordinates.
+// All 24 loads are issued before any arithmetic, and the shifts are formed with factors 0 or 1
+// (exact products) rather than data-dependent selects, so that no load ends up behind a branch
+// and a full wait; the sums are those of the selects, in the same order.
 __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
                                                 const double* __restrict__ rv, const int n[8],
                                                 int wrap, double r[8][3]) {
-  double p0[3];
+  double p[8][3];
 #pragma unroll
-  for (int j = 0; j < 3; ++j) p0[j] = pos[3 * (int64_t)n[0] + j];
+  for (int k = 0; k < 8; ++k)
 #pragma unroll
-  for (int j = 0; j < 3; ++j) r[0][j] = p0[j];
+    for (int j = 0; j < 3; ++j) p[k][j] = pos[3 * (int64_t)n[k] + j];
+  const double fx = (double)(wrap & 1), fy = (double)((wrap >> 1) & 1),
+               fz = (double)((wrap >> 2) & 1);
+  double sx[3], sy[3], sz[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    sx[j] = fx * rv[0 + j];
+    sy[j] = fy * rv[3 + j];
+    sz[j] = fz * rv[6 + j];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) r[0][j] = 0.0;
 #pragma unroll
   for (int k = 1; k < 8; ++k) {
-    const bool sx = kOff[k][0] && (wrap & 1);
-    const bool sy = kOff[k][1] && (wrap & 2);
-    const bool sz = kOff[k][2] && (wrap & 4);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const double sh = ((sx ? rv[0 + j] : 0.0) + (sy ? rv[3 + j] : 0.0)) + (sz ? rv[6 + j] : 0.0);
-      const double dvec = (pos[3 * (int64_t)n[k] + j] - p0[j]) + sh;
-      r[k][j] = p0[j] + dvec;
+      const double sh = ((kOff[k][0] ? sx[j] : 0.0) + (kOff[k][1] ? sy[j] : 0.0)) +
+                        (kOff[k][2] ? sz[j] : 0.0);
+      r[k][j] = (p[k][j] - p[0][j]) + sh;
     }
   }
 }
@@ -128,14 +142,16 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   int n[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) n[k] = cnode[w * 8 + k];
+  // sums start at -0.0, the identity of addition ((-0.0) + x == x for every x), so that the
+  // first term of each needs no add
   double acc[10];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int q = 0; q < 10; ++q) acc[q] = -0.0;
   double gacc[8][3];
 #pragma unroll
   for (int v = 0; v < 8; ++v)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gacc[v][j] = 0.0;
+    for (int j = 0; j < 3; ++j) gacc[v][j] = -0.0;
   const bool valid = n[0] >= 0;
   const int wrap = valid ? cwrap[w] : 0;
   if (valid) {
@@ -169,7 +185,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
 #pragma unroll
         for (int v = 0; v < 8; ++v)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) gs[v][j] = 0.0;
+          for (int j = 0; j < 3; ++j) gs[v][j] = -0.0;
         const double es = ts[10] + state_energy<kModel, true, true>(ts, r, gs);
         if (s == 0) {
           emin = es;
@@ -200,13 +216,10 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       // virial: vtens[k][l] = sum_v g_v[k] (r_v - r_0)[l]
 #pragma unroll
       for (int v = 1; v < 8; ++v) {
-        double d[3];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) d[l] = r[v][l] - r[0][l];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] += gacc[v][k] * d[l];
+          for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] += gacc[v][k] * r[v][l];
       }
     }
   }
@@ -215,12 +228,14 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
     // (corners 0, 1, 2, 4) when that slot holds the -z neighbour (wrap bit 3); the lower faces
     // go to their class planes, the upper faces that were not folded (wrap bit 4 clear) to the
     // wave's compact slots, in lane order.
+    const bool fold = (wrap & 8) != 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double t = __shfl_up(gacc[kUpperCorner[q]][j], 1, 64);
-        if (wrap & 8) gacc[kLowerCorner[q]][j] += t;
+        const double lo = gacc[kLowerCorner[q]][j];
+        gacc[kLowerCorner[q]][j] = fold ? lo + t : lo;
       }
     }
     if (valid) {
