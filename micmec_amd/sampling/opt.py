@@ -180,23 +180,34 @@ class ConjugateGradientSearch(object):
             if dphi0 == 0.0:
                 return True
         f0 = self.f
-        ftol = 8.0 * np.finfo(float).eps * max(abs(f0), np.finfo(float).tiny)
+        scale = max(abs(f0), np.finfo(float).tiny)
+        ftol = 8.0 * np.finfo(float).eps * scale
+        # Energies within fnoise of phi(0) count as level with it: near a minimum the energy is
+        # resolved only to the rounding of its strains (differences of numbers close to 1,
+        # about 1e-13 relative per cell), so a trial there is judged by its slope; a level
+        # trial is kept only when no trial decreases the energy.
+        fnoise = 1e-12 * scale
         # bracket [lo, hi] of the minimum along d: lo has phi'(lo) < 0 and phi(lo) <= phi(0),
         # hi has phi'(hi) > 0 or no decrease; the next trial is the secant zero of phi' in the
         # bracket (exact on quadratics), kept a hundredth of the width away from its ends
         lo_a, lo_dp = 0.0, dphi0
         hi_a, hi_dp = None, None
         a = self.alpha
-        best = None
+        best = near = None
         for _ in range(self.max_ls):
             fa, dp = E.trial(a)
-            ok = isfinite(fa) and isfinite(dp) and fa <= f0 + ftol
+            finite = isfinite(fa) and isfinite(dp)
+            ok = finite and fa <= f0 + ftol
+            level = finite and fa <= f0 + fnoise
             if ok and (best is None or abs(dp) < abs(best[2])):
                 best = (a, fa, dp)
                 E.keep()
-            if ok and abs(dp) <= self.curvature * abs(dphi0):
+            elif level and best is None and (near is None or abs(dp) < abs(near[2])):
+                near = (a, fa, dp)
+                E.keep()
+            if level and abs(dp) <= self.curvature * abs(dphi0):
                 break
-            if ok and dp < 0.0:
+            if level and dp < 0.0:
                 prev_a, prev_dp = lo_a, lo_dp
                 lo_a, lo_dp = a, dp
             else:
@@ -219,6 +230,8 @@ class ConjugateGradientSearch(object):
             if self.max_step is not None:
                 a_new = min(a_new, self.max_step)
             a = a_new
+        if best is None:
+            best = near
         if best is None:
             return False
         a, fa, dp = best

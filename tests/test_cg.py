@@ -97,3 +97,30 @@ def test_cg_optimizer_reaches_oracle_minimum(model):
     # the state of the system is the optimised geometry
     e_sys = mmf.compute()
     assert abs(e_sys - opt.epot) <= 1e-12 * abs(opt.epot)
+
+
+def test_cg_search_converges_through_energy_noise():
+    """Near a minimum the energy is known only to the rounding of its strains (about 1e-13
+    relative per cell): on a quadratic whose energy carries 1e-14 relative noise (a fixed
+    function of x, as rounding is) the search must keep following the gradient to |g| < 1e-10
+    instead of giving up when no trial lowers the energy beyond the noise."""
+    import zlib
+    rng = np.random.default_rng(0)
+    n = 40
+    A = rng.normal(size=(n, n))
+    A = A @ A.T / n + np.diag(np.logspace(0, 4, n))
+    b = rng.normal(size=n)
+
+    def fun(x, g=True):
+        f = 0.5 * x @ A @ x - b @ x + 10.0
+        h = zlib.crc32(np.ascontiguousarray(x).tobytes()) / 2.0 ** 32 - 0.5
+        return f * (1.0 + 2e-14 * h), A @ x - b
+
+    cg = ConjugateGradientSearch(np.zeros(n), fun)
+    cg.initialize()
+    for _ in range(3000):
+        assert cg.propagate(), np.linalg.norm(cg.g)
+        if np.linalg.norm(cg.g) < 1e-10:
+            break
+    assert np.linalg.norm(cg.g) < 1e-10
+    assert np.allclose(cg.x, np.linalg.solve(A, b), atol=1e-12)
