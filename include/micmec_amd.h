@@ -69,6 +69,17 @@ int mm_field_create(mm_field** out, int device, int model, int64_t ncells, int64
 int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double* energy,
                      double* gpos, double* vtens, int accumulate, void* stream);
 
+/* mm_field_compute without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * vertex-gradient records and block partials go to a static workspace of 64 MiB that every
+ * GPU holds for the life of the process, instead of the field's own buffers. No counterpart in
+ * the reference. Same arguments, results (bit for bit) and semantics as mm_field_compute;
+ * fields needing more than the workspace (about 400,000 cells) are refused with MM_ERR_ARG.
+ * Calls that overlap on one GPU (other streams, other threads) share the workspace and race:
+ * serialise them. */
+int mm_field_compute_static(mm_field* f, const double* pos, const double* rvecs, double* energy,
+                            double* gpos, double* vtens, int accumulate, void* stream);
+
 /* Synchronise `stream`, report and clear the NaN flags of the computes issued since the last
  * check (MM_NAN_* bits in *flags). Returns MM_ERR_NAN if any flag was set
  * (micmec/pes/mmff.py:135-148). */

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,14 @@
 namespace {
 thread_local std::string g_last_error;
 }
+
+// Workspace of mm_field_compute_static: the vertex-gradient records (12 planes of the field's
+// S slots) followed by the block partials (kPartialStride rows of W / kCellBlock) of one force
+// evaluation. Every GPU that loads the library holds its own copy, at its own address, until
+// the process ends; in host code the name only stands in for it (hipGetSymbolAddress gives
+// the device address, once per device, below). All calls on one GPU share it.
+constexpr size_t kStaticScratchDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
 
 namespace mm {
 
@@ -547,6 +556,65 @@ int mm_field_compute(mm_field* f, const double* pos, const double* rvecs, double
   a.pos = pos;
   a.rvecs = rvecs;
   field_args(f, a);
+  a.energy = energy;
+  a.gpos = gpos;
+  a.vtens = vtens;
+  a.accumulate = accumulate;
+  hipError_t e = launch_force(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_error(e, "force kernels");
+  return MM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of the static workspace on `device` (which must be current), looked up on the
+// first call there and kept.
+int static_scratch(int device, double** out) {
+  if (device < 0 || device >= kScratchDevices)
+    return mm::set_error(MM_ERR_ARG, "device %d out of range for the static workspace", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return mm::hip_error(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return MM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mm_field_compute_static(mm_field* f, const double* pos, const double* rvecs, double* energy,
+                            double* gpos, double* vtens, int accumulate, void* stream) {
+  if (!f) return set_error(MM_ERR_ARG, "field is NULL");
+  if (!energy) return set_error(MM_ERR_ARG, "energy output is required");
+  if (f->N > 0 && !pos) return set_error(MM_ERR_ARG, "pos is NULL");
+  if (!rvecs) return set_error(MM_ERR_ARG, "rvecs is NULL");
+  const int64_t nb = f->W / kCellBlock;
+  const int64_t need = 12 * f->S + (int64_t)kPartialStride * nb;
+  if (need > (int64_t)kStaticScratchDoubles)
+    return set_error(MM_ERR_ARG, "this field needs %lld doubles of workspace (%lld cells), the "
+                     "static workspace holds %lld; use mm_field_compute",
+                     (long long)need, (long long)f->C, (long long)kStaticScratchDoubles);
+  DeviceGuard guard(f->device);
+  double* ws = nullptr;
+  int rc = static_scratch(f->device, &ws);
+  if (rc) return rc;
+  ForceArgs a;
+  a.pos = pos;
+  a.rvecs = rvecs;
+  field_args(f, a);
+  a.gcell = ws;
+  a.partials = ws + 12 * f->S;
   a.energy = energy;
   a.gpos = gpos;
   a.vtens = vtens;
