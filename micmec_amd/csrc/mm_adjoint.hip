@@ -176,23 +176,36 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
   const double* rv = rvecs + 9 * (int64_t)sys;
   const double* drv = drvecs ? drvecs + 9 * (int64_t)sys : nullptr;
 
-  // vertex of this lane: r_0 = p_0, r_k = p_0 + ((p_k - p_0) + shift_k) (mmff.py:347-371)
+  // vertex of this lane: r_0 = p_0, r_k = p_0 + ((p_k - p_0) + shift_k) (mmff.py:347-371).
+  // Every load is issued unconditionally (a padding slot reads node 0 and discards it) and the
+  // shifts use factors 0 or 1 (exact products), so that no load sits behind a branch.
+  const int64_t nn = valid ? n : 0;
+  double pv[3], dv[3] = {0.0, 0.0, 0.0}, rvv[9], drvv[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) pv[j] = pos[3 * nn + j];
+  if (dpos) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dv[j] = dpos[3 * nn + j];
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    rvv[q] = rv[q];
+    drvv[q] = drv ? drv[q] : 0.0;
+  }
   T p[3];
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
-    p[j] = make<T>(valid ? pos[3 * (int64_t)n + j] : 0.0,
-                   (valid && dpos) ? dpos[3 * (int64_t)n + j] : 0.0);
+  for (int j = 0; j < 3; ++j) p[j] = make<T>(valid ? pv[j] : 0.0, valid ? dv[j] : 0.0);
   T r[3], r0[3];
   {
-    const bool sx = (b & 1) && (wrap & 1);
-    const bool sy = (b & 2) && (wrap & 2);
-    const bool sz = (b & 4) && (wrap & 4);
+    const int moved = b & wrap;
+    const double fx = (double)(moved & 1), fy = (double)((moved >> 1) & 1),
+                 fz = (double)((moved >> 2) & 1);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       r0[j] = bc(p[j], lead_lane);
-      const T s0 = make<T>(sx ? rv[0 + j] : 0.0, (sx && drv) ? drv[0 + j] : 0.0);
-      const T s1 = make<T>(sy ? rv[3 + j] : 0.0, (sy && drv) ? drv[3 + j] : 0.0);
-      const T s2 = make<T>(sz ? rv[6 + j] : 0.0, (sz && drv) ? drv[6 + j] : 0.0);
+      const T s0 = make<T>(fx * rvv[0 + j], fx * drvv[0 + j]);
+      const T s1 = make<T>(fy * rvv[3 + j], fy * drvv[3 + j]);
+      const T s2 = make<T>(fz * rvv[6 + j], fz * drvv[6 + j]);
       const T dvec = (p[j] - r0[j]) + ((s0 + s1) + s2);
       r[j] = b ? r0[j] + dvec : p[j];
     }
@@ -376,10 +389,9 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
       stage(vrow, 1 + 3 * k + l, sum8(g[k] * (r[l] - r0[l])), lead, valid);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const bool moves = ((b >> d) & 1) && ((wrap >> d) & 1);
+    const double moves = (double)(((b & wrap) >> d) & 1);
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      stage(vrow, 10 + 3 * d + j, sum8(moves ? g[j] : make<T>(0.0, 0.0)), lead, valid);
+    for (int j = 0; j < 3; ++j) stage(vrow, 10 + 3 * d + j, sum8(moves * g[j]), lead, valid);
   }
   // dE/dkT rides along in column kSysQ of the same flush, into its parameter rows
   stage(vrow, kSysQ, dkt, lead, valid);
