@@ -12,8 +12,8 @@ import torch
 from micmec_amd import units
 from micmec_amd.pes import ForcePartMechanical, MicMecForceField, ForcePartPressure
 from micmec_amd.system import System
-from micmec_amd.topology import Grid
-from micmec_amd.workloads import make_system
+from micmec_amd.topology import Grid, build_system, build_type
+from micmec_amd.workloads import CELL_MASS, cubic_elasticity, make_system, type_a, type_b
 from oracle import mmff as omm
 
 from golden.make_golden import load_system
@@ -81,6 +81,43 @@ def test_reference_fixtures(fn):
 def test_synthetic_multistate(shape, p_b, p_hole, seed):
     print("synthetic", shape, p_b, p_hole, seed, flush=True)
     s = make_system(shape, p_b=p_b, p_hole=p_hole, seed=seed, noise=0.1 * units.angstrom)
+    for model in ("corner8", "original"):
+        e, g, v = gpu_eval(s, model)
+        e0, g0, v0 = oracle_eval(s, model)
+        assert rel(e, e0) < TOL, (model, e, e0)
+        assert rel(g, g0) < TOL, (model, rel(g, g0))
+        assert rel(v, v0) < TOL, (model, rel(v, v0))
+
+
+def many_state_system(shape, ns, seed):
+    """Seeded grid of type A (one state), type B (two states) and a type C of ``ns`` states in
+    about equal shares, 10% of the cells empty. C's states differ in h0 by under 1.5% and in
+    free energy by 1e-4 Ha steps (kT at 300 K is 9.5e-4 Ha), so every state carries weight."""
+    rng = np.random.default_rng(seed)
+    grid = rng.integers(1, 4, size=shape).astype(np.int64)
+    grid[rng.random(shape) < 0.1] = 0
+    h0 = 20.8 * units.angstrom * np.identity(3)
+    scales = [1.0, 0.995, 1.004, 0.99, 1.008, 0.985][:ns]
+    C = cubic_elasticity(35.0, 19.0, 8.0)
+    tc = build_type("C", CELL_MASS, [s * h0 for s in scales],
+                    [(1.0 - 0.05 * i) * C for i in range(ns)],
+                    free_energy=[1e-4 * i for i in range(ns)],
+                    effective_temp=300.0 * units.kelvin)
+    data = {1: type_a(), 2: type_b(), 3: tc}
+    out = build_system({t: data[t] for t in sorted({int(t) for t in np.unique(grid) if t})},
+                       grid, pbc=True)
+    out["pos"] = out["pos"] + rng.normal(0.0, 0.1 * units.angstrom, out["pos"].shape)
+    return System.from_dict(out)
+
+
+@pytest.mark.parametrize("ns", [3, 4, 5])
+def test_many_state_types(ns):
+    """Types of three to five states beside one- and two-state types: the multistate lanes loop
+    over more states (the energy-only instance keeps them all in LDS). Against the oracle
+    (mmff.py:386-398)."""
+    print("many states", ns, flush=True)
+    s = many_state_system((9, 7, 8), ns, seed=40 + ns)
+    assert len(s.params["type3/free_energy"]) == ns
     for model in ("corner8", "original"):
         e, g, v = gpu_eval(s, model)
         e0, g0, v0 = oracle_eval(s, model)
