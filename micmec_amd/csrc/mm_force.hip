@@ -108,7 +108,8 @@ __device__ constexpr int kUpperCorner[4] = {3, 5, 6, 7};
 
 // One lane per cell. Energies of all states of the block's cell type, combined by the
 // log-sum-exp of mmff.py:386-398; the vertex gradient of the combined energy goes to gcell and
-// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials. A block holds
+// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials (with store = 0
+// the gradient feeds only the virial: no face fold and no records). A block holds
 // one cell type; single-state and multistate blocks run in separate instances (kMulti), each
 // launched over its own range of blocks (blk0 + blockIdx.x) when the work order allows, so that
 // the single-state instance does not carry the registers of the state combination.
@@ -120,7 +121,8 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
             int32_t max_states, int64_t W, const int32_t* __restrict__ cbase, int64_t S,
-            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks) {
+            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks,
+            int store) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[(kMulti && !kGrad) ? kMaxStates * kCellBlock : 1];
   // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
@@ -223,7 +225,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       }
     }
   }
-  if (kGrad) {
+  if (kGrad && store) {
     // The upper face (corners 3, 5, 6, 7) of the slot below is added to this slot's lower face
     // (corners 0, 1, 2, 4) when that slot holds the -z neighbour (wrap bit 3); the lower faces
     // go to their class planes, the upper faces that were not folded (wrap bit 4 clear) to the
@@ -424,7 +426,7 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(NBL), dim3(kCellBlock), 0, S,   \
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
                      a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
-                     a.partials, B0, nb)
+                     a.partials, B0, nb, a.store_records)
 #define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
   do {                                                                                       \
     if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \

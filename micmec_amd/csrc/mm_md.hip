@@ -30,6 +30,8 @@
 namespace mm {
 
 constexpr int kNodeBlock = 256;   // nodes per block (system boundaries are block aligned)
+// What a node gather does with the face-class records (gather_kick_kernel).
+constexpr int kGatherNone = 0, kGatherKeep = 1, kGatherStore = 2;
 constexpr int kNodeThreads = 64;  // one wave per block, kNodeBlock / kNodeThreads nodes a lane
 // node sums: [0] ekin [1..9] sum m v(x)v [10] sum g^2 [11] sum d^2 [12] nan
 // [13] ekin before the Langevin post update [14] ekin after the Langevin pre update
@@ -238,13 +240,17 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // With kScaleV instead (a barostat call, npt.py:704-735), the pending velocity transforms are
 // applied (the barostat's exp(-(v_g + tr(v_g)/ndof) dt / 2), known before its force) and the
 // node sums of the transformed velocities taken, in the same pass.
+// gmode: kGatherStore sums the records and stores gpos; kGatherKeep sums them for the kick only
+// (the step overwrites gpos before anything reads it); kGatherNone (kScaleV only) reads no
+// records: the force evaluation fed only the energy and the virial.
 // One node per thread (the gathers need every wave in flight that they can get).
 template <bool kKick, bool kLT, bool kScaleV = false>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t RS, double* __restrict__ gpos, double* __restrict__ v,
                    const double* __restrict__ m, const double* __restrict__ S0,
-                   const int32_t* __restrict__ blk_sys, double* __restrict__ partials) {
+                   const int32_t* __restrict__ blk_sys, double* __restrict__ partials,
+                   int gmode) {
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
   constexpr int NA = kLT ? 14 : 13;
   double acc[NA];
@@ -254,21 +260,23 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
     if (n < N) {
     double g[3] = {0.0, 0.0, 0.0};
+    if (gmode != kGatherNone) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
-      if (e >= 0) {
-        const int64_t w = e >> 2;
-        const int p = e & 3;
+      for (int k = 0; k < 8; ++k) {
+        const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
+        if (e >= 0) {
+          const int64_t w = e >> 2;
+          const int p = e & 3;
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
+          for (int j = 0; j < 3; ++j)
+            g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
+        }
       }
-    }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      gpos[3 * n + j] = g[j];
-      if (g[j] != g[j]) acc[12] = 1.0;
+      for (int j = 0; j < 3; ++j) {
+        if (gmode == kGatherStore) gpos[3 * n + j] = g[j];
+        if (g[j] != g[j]) acc[12] = 1.0;
+      }
     }
     if (kKick) {
 #pragma clang fp contract(off)
@@ -706,7 +714,10 @@ __device__ void take_force(double* S, const double (&f)[10], bool with_vtens) {
   const double pv = P * S[MM_S_VOLUME];
   S[MM_S_EPOT] = P != 0.0 ? f[0] + S[MM_S_VOLUME] * P : f[0];
   if (with_vtens) {
-    for (int q = 0; q < 9; ++q) S[MM_S_VTENS + q] = f[1 + q];
+    for (int q = 0; q < 9; ++q) {
+      S[MM_S_VTENS + q] = f[1 + q];
+      if (f[1 + q] != f[1 + q]) S[MM_S_NAN] = 1.0;
+    }
     if (P != 0.0) {
       S[MM_S_VTENS + 0] += pv;
       S[MM_S_VTENS + 4] += pv;
@@ -1000,9 +1011,11 @@ struct mm_md {
   unsigned int* counters = nullptr;  // [nsys] last-block counters (kept at zero between calls)
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraph_t graph = nullptr;         // one step
-  hipGraphExec_t exec = nullptr;
-  int graph_split = 0;                // the field's block split the graph was captured with
+  // One step, and (barostat runs) the same step whose last force evaluation feeds only the
+  // energy and virial: replayed for every step of a call but the last.
+  hipGraph_t graph = nullptr, graph_lean = nullptr;
+  hipGraphExec_t exec = nullptr, exec_lean = nullptr;
+  int graph_split = 0;                // the field's block split the graphs were captured with
 };
 
 namespace {
@@ -1085,16 +1098,23 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
 }
 
 // force evaluation: cells, then gather (+ second half kick when kick, + the Langevin
-// thermostat's post update when lt; + the pending velocity transforms when scale_v)
-hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = false) {
+// thermostat's post update when lt; + the pending velocity transforms when scale_v). gmode
+// (gather_kick_kernel): kGatherNone (scale_v only) leaves out the records altogether, so the
+// cell pass skips them too.
+hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = false,
+                 int gmode = kGatherStore) {
   ForceArgs a = force_args(md);
+  if (gmode == kGatherNone) {
+    if (kick) return hipErrorInvalidValue;
+    a.store_records = 0;
+  }
   hipError_t e = launch_cells(a, s);
   if (e != hipSuccess) return e;
   if (md->N > 0) {
 #define MM_GATHER(K, T, V)                                                                   \
   hipLaunchKernelGGL((gather_kick_kernel<K, T, V>), dim3(md->nbn), dim3(kNodeBlock), 0, s,    \
                      a.ncell, a.gcell, md->N, a.rstride, md->gpos, md->vel, md->masses, md->S,      \
-                     md->node_blk_sys, md->npart)
+                     md->node_blk_sys, md->npart, gmode)
     if (kick && lt)
       MM_GATHER(true, true, false);
     else if (kick)
@@ -1117,23 +1137,29 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
 
 // The barostat call MTKBarostat.baro after its first scalar phase: positions scaled, force,
 // velocities transformed in the gather, then the second barostat-velocity update.
-hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s) {
+hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, int gmode = kGatherStore) {
   MM_TRY(node(md, kScalePos, s));
-  MM_TRY(force(md, false, false, s, true));
+  MM_TRY(force(md, false, false, s, true, gmode));
   MM_TRY(scalar(md, end_phase, 1, s));
   return hipSuccess;
 }
 
-hipError_t enqueue_step(mm_md* md, hipStream_t s) {
+// One step. lean (barostat runs): a step that another step follows before anything reads the
+// node gradients. The gradient of its main force evaluation is then used only by the kick in
+// the gather, and that of the barostat's post call by nothing (the next step's pre call
+// evaluates the force again before its kick, verlet.py:141-143 with npt.py:616-620), so those
+// evaluations keep their gradients out of HBM; positions, velocities, energies, virials and
+// every scalar are those of the full step, bit for bit.
+hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false) {
   const bool lt = md->thermo == MM_THERMO_LANGEVIN;
   const int kick_drift = lt ? kKickDriftLT : kKickDrift;
   if (md->baro) {
     MM_TRY(scalar(md, kPhBaroPre, 1, s));
     MM_TRY(baro_call(md, kPhBaroEndPre, s));
     MM_TRY(node(md, kick_drift, s));
-    MM_TRY(force(md, true, lt, s));
+    MM_TRY(force(md, true, lt, s, false, lean ? kGatherKeep : kGatherStore));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
-    MM_TRY(baro_call(md, kPhBaroEndPost, s));
+    MM_TRY(baro_call(md, kPhBaroEndPost, s, lean ? kGatherNone : kGatherStore));
   } else if (md->thermo == MM_THERMO_NHC) {
     MM_TRY(scalar(md, kPhNvtPre, 1, s));
     MM_TRY(node(md, kKickDrift, s));
@@ -1168,21 +1194,23 @@ int end(mm_md* md, void* stream) {
 namespace {
 
 // Capture one step into an executable graph (replayed once per step).
-hipError_t capture_step(mm_md* md) {
+hipError_t capture_step(mm_md* md, bool lean, hipGraph_t* graph, hipGraphExec_t* exec) {
   hipError_t e = hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  const hipError_t e2 = enqueue_step(md, md->stream);
-  e = hipStreamEndCapture(md->stream, &md->graph);
+  const hipError_t e2 = enqueue_step(md, md->stream, lean);
+  e = hipStreamEndCapture(md->stream, graph);
   if (e2 != hipSuccess) e = e2;
-  if (e == hipSuccess) e = hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
   return e;
 }
 
 void drop_graph(mm_md* md) {
   if (md->exec) (void)hipGraphExecDestroy(md->exec);
   if (md->graph) (void)hipGraphDestroy(md->graph);
-  md->exec = nullptr;
-  md->graph = nullptr;
+  if (md->exec_lean) (void)hipGraphExecDestroy(md->exec_lean);
+  if (md->graph_lean) (void)hipGraphDestroy(md->graph_lean);
+  md->exec = md->exec_lean = nullptr;
+  md->graph = md->graph_lean = nullptr;
 }
 
 }  // namespace
@@ -1309,12 +1337,20 @@ int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream) {
     if (md->exec && md->graph_split != md->field->split) drop_graph(md);
     if (!md->exec) {
       md->graph_split = md->field->split;
-      e = capture_step(md);
+      e = capture_step(md, false, &md->graph, &md->exec);
       if (e != hipSuccess) return hip_error(e, "capturing the MD step");
     }
-    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = hipGraphLaunch(md->exec, md->stream);
+    // every step but the last is followed by another before the call returns
+    const bool lean = md->baro != 0 && nsteps > 1;
+    if (lean && !md->exec_lean) {
+      e = capture_step(md, true, &md->graph_lean, &md->exec_lean);
+      if (e != hipSuccess) return hip_error(e, "capturing the MD step");
+    }
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i)
+      e = hipGraphLaunch(lean && i + 1 < nsteps ? md->exec_lean : md->exec, md->stream);
   } else {
-    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = enqueue_step(md, md->stream);
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i)
+      e = enqueue_step(md, md->stream, md->baro != 0 && i + 1 < nsteps);
   }
   if (e != hipSuccess) return hip_error(e, "mm_md_step");
   return end(md, stream);
