@@ -354,8 +354,9 @@ int mm_md_init_properties(mm_md* md, void* stream);
  * between multistate and single-state cell blocks. With a barostat, every step but the last
  * leaves gpos alone after its pre-barostat force evaluation (the gradients of its main and
  * post-barostat evaluations are read by nothing before the next step's pre-barostat
- * evaluation); after the call gpos, like every other array and scalar, holds exactly what
- * nsteps calls of one step would leave. */
+ * evaluation), and the closing scalar update of each step but the last runs in one kernel with
+ * the opening one of the next; after the call gpos, like every other array and scalar, holds
+ * exactly what nsteps calls of one step would leave. */
 int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream);
 
 /* Apply pending velocity/position transforms so that pos and vel hold the reference state. */

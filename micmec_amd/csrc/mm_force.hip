@@ -427,10 +427,12 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
                      a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
                      a.partials, B0, nb, a.store_records)
+  // The single-state instance, which takes longer, is launched first, on the caller's stream:
+  // it then starts right behind the preceding kernel instead of after the side stream's.
 #define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
   do {                                                                                       \
-    if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \
     if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, stream, s0, sn);                         \
+    if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \
   } while (0)
   if (a.model == 0) {
     if (grad) MM_LAUNCH_PAIR(0, true); else MM_LAUNCH_PAIR(0, false);

@@ -62,6 +62,7 @@ enum Phase {
   kPhBaroEndPost = 9,    // force + node sums; v_g update; econs; properties
   kPhRmsd = 10,          // rmsd_gpos / rmsd_delta
   kPhApplied = 11,       // pending transforms were applied
+  kPhBaroEndPostPre = 12,  // kPhBaroEndPost, then the next step's kPhBaroPre (steps in a row)
 };
 
 // ----------------------------------------------------------------------------- node kernels
@@ -770,12 +771,13 @@ __device__ void reduce_blocks(const double* __restrict__ part, int nb,
 
 __device__ __forceinline__ bool phase_uses_force(int phase) {
   return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
-         phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost;
+         phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
+         phase == kPhBaroEndPostPre;
 }
 __device__ __forceinline__ bool phase_uses_nodes(int phase) {
   return phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
          phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
-         phase == kPhRmsd;
+         phase == kPhBaroEndPostPre || phase == kPhRmsd;
 }
 
 // The scalar update of one phase on the scalar block S of system sys.
@@ -982,7 +984,15 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
   double* Sg = S0 + (int64_t)sys * MM_S_COUNT;
   for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sl[q] = Sg[q];
   __syncthreads();
-  if (threadIdx.x == 0) run_phase(phase, Sl, f, nd, with_vtens, sys);
+  if (threadIdx.x == 0) {
+    if (phase == kPhBaroEndPostPre) {
+      // the end of one step and the start of the next, on the same copy of the block
+      run_phase(kPhBaroEndPost, Sl, f, nd, with_vtens, sys);
+      run_phase(kPhBaroPre, Sl, f, nd, with_vtens, sys);
+    } else {
+      run_phase(phase, Sl, f, nd, with_vtens, sys);
+    }
+  }
   __syncthreads();
   for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sg[q] = Sl[q];
 }
@@ -1011,10 +1021,11 @@ struct mm_md {
   unsigned int* counters = nullptr;  // [nsys] last-block counters (kept at zero between calls)
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  // One step, and (barostat runs) the same step whose last force evaluation feeds only the
-  // energy and virial: replayed for every step of a call but the last.
-  hipGraph_t graph = nullptr, graph_lean = nullptr;
-  hipGraphExec_t exec = nullptr, exec_lean = nullptr;
+  // One step; barostat runs with several steps in a call replay, after the opening scalar
+  // phase, the lean step without it (which opens the next step) and then the full step
+  // without it.
+  hipGraph_t graph = nullptr, graph_lean = nullptr, graph_last = nullptr;
+  hipGraphExec_t exec = nullptr, exec_lean = nullptr, exec_last = nullptr;
   int graph_split = 0;                // the field's block split the graphs were captured with
 };
 
@@ -1084,7 +1095,7 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
 bool reduces(int phase) {
   return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
          phase == kPhMainEndNpt || phase == kPhInitProps || phase == kPhBaroEndPre ||
-         phase == kPhBaroEndPost || phase == kPhRmsd;
+         phase == kPhBaroEndPost || phase == kPhBaroEndPostPre || phase == kPhRmsd;
 }
 
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
@@ -1150,16 +1161,20 @@ hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, int gmode = kGathe
 // evaluates the force again before its kick, verlet.py:141-143 with npt.py:616-620), so those
 // evaluations keep their gradients out of HBM; positions, velocities, energies, virials and
 // every scalar are those of the full step, bit for bit.
-hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false) {
+// Barostat runs also split a call's steps at their first scalar phase: with pre = false the
+// step leaves out its opening kPhBaroPre (the step before ran it), and a lean step ends with
+// kPhBaroEndPostPre, which opens the next step too.
+hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = true) {
   const bool lt = md->thermo == MM_THERMO_LANGEVIN;
   const int kick_drift = lt ? kKickDriftLT : kKickDrift;
   if (md->baro) {
-    MM_TRY(scalar(md, kPhBaroPre, 1, s));
+    if (pre) MM_TRY(scalar(md, kPhBaroPre, 1, s));
     MM_TRY(baro_call(md, kPhBaroEndPre, s));
     MM_TRY(node(md, kick_drift, s));
     MM_TRY(force(md, true, lt, s, false, lean ? kGatherKeep : kGatherStore));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
-    MM_TRY(baro_call(md, kPhBaroEndPost, s, lean ? kGatherNone : kGatherStore));
+    MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s,
+                     lean ? kGatherNone : kGatherStore));
   } else if (md->thermo == MM_THERMO_NHC) {
     MM_TRY(scalar(md, kPhNvtPre, 1, s));
     MM_TRY(node(md, kKickDrift, s));
@@ -1194,10 +1209,11 @@ int end(mm_md* md, void* stream) {
 namespace {
 
 // Capture one step into an executable graph (replayed once per step).
-hipError_t capture_step(mm_md* md, bool lean, hipGraph_t* graph, hipGraphExec_t* exec) {
+hipError_t capture_step(mm_md* md, bool lean, bool pre, hipGraph_t* graph,
+                        hipGraphExec_t* exec) {
   hipError_t e = hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  const hipError_t e2 = enqueue_step(md, md->stream, lean);
+  const hipError_t e2 = enqueue_step(md, md->stream, lean, pre);
   e = hipStreamEndCapture(md->stream, graph);
   if (e2 != hipSuccess) e = e2;
   if (e == hipSuccess) e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
@@ -1209,8 +1225,10 @@ void drop_graph(mm_md* md) {
   if (md->graph) (void)hipGraphDestroy(md->graph);
   if (md->exec_lean) (void)hipGraphExecDestroy(md->exec_lean);
   if (md->graph_lean) (void)hipGraphDestroy(md->graph_lean);
-  md->exec = md->exec_lean = nullptr;
-  md->graph = md->graph_lean = nullptr;
+  if (md->exec_last) (void)hipGraphExecDestroy(md->exec_last);
+  if (md->graph_last) (void)hipGraphDestroy(md->graph_last);
+  md->exec = md->exec_lean = md->exec_last = nullptr;
+  md->graph = md->graph_lean = md->graph_last = nullptr;
 }
 
 }  // namespace
@@ -1334,23 +1352,27 @@ int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream) {
   if (use_graph) {
     // the launch ranges of the cell pass follow the field's block split: a new split (new
     // state counts through mm_field_set_params) needs a new graph
-    if (md->exec && md->graph_split != md->field->split) drop_graph(md);
-    if (!md->exec) {
-      md->graph_split = md->field->split;
-      e = capture_step(md, false, &md->graph, &md->exec);
-      if (e != hipSuccess) return hip_error(e, "capturing the MD step");
-    }
+    if (md->graph_split != md->field->split) drop_graph(md);
+    md->graph_split = md->field->split;
     // every step but the last is followed by another before the call returns
     const bool lean = md->baro != 0 && nsteps > 1;
-    if (lean && !md->exec_lean) {
-      e = capture_step(md, true, &md->graph_lean, &md->exec_lean);
-      if (e != hipSuccess) return hip_error(e, "capturing the MD step");
+    if (!lean && !md->exec) e = capture_step(md, false, true, &md->graph, &md->exec);
+    if (lean && !md->exec_lean) e = capture_step(md, true, false, &md->graph_lean, &md->exec_lean);
+    if (lean && e == hipSuccess && !md->exec_last)
+      e = capture_step(md, false, false, &md->graph_last, &md->exec_last);
+    if (e != hipSuccess) return hip_error(e, "capturing the MD step");
+    if (lean) {
+      e = scalar(md, kPhBaroPre, 1, md->stream);
+      for (int i = 0; i + 1 < nsteps && e == hipSuccess; ++i)
+        e = hipGraphLaunch(md->exec_lean, md->stream);
+      if (e == hipSuccess) e = hipGraphLaunch(md->exec_last, md->stream);
+    } else {
+      for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = hipGraphLaunch(md->exec, md->stream);
     }
-    for (int i = 0; i < nsteps && e == hipSuccess; ++i)
-      e = hipGraphLaunch(lean && i + 1 < nsteps ? md->exec_lean : md->exec, md->stream);
   } else {
+    const bool lean = md->baro != 0 && nsteps > 1;
     for (int i = 0; i < nsteps && e == hipSuccess; ++i)
-      e = enqueue_step(md, md->stream, md->baro != 0 && i + 1 < nsteps);
+      e = enqueue_step(md, md->stream, lean && i + 1 < nsteps, !lean || i == 0);
   }
   if (e != hipSuccess) return hip_error(e, "mm_md_step");
   return end(md, stream);
