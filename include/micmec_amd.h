@@ -351,12 +351,12 @@ int mm_md_init_properties(mm_md* md, void* stream);
 /* Enqueue nsteps integration steps (verlet.py:140-166 with the hooks' pre/post). With
  * use_graph != 0 one step is captured once into a HIP graph and replayed; the graph is
  * captured again when new state counts (mm_field_set_params) have changed the field's split
- * between multistate and single-state cell blocks. With a barostat, every step but the last
- * leaves gpos alone after its pre-barostat force evaluation (the gradients of its main and
- * post-barostat evaluations are read by nothing before the next step's pre-barostat
- * evaluation), and the closing scalar update of each step but the last runs in one kernel with
- * the opening one of the next; after the call gpos, like every other array and scalar, holds
- * exactly what nsteps calls of one step would leave. */
+ * between multistate and single-state cell blocks. The closing scalar update of each step but
+ * the last runs in one kernel with the opening one of the next (barostat and Nose-Hoover chain
+ * runs), and with a barostat every step but the last leaves gpos alone after its pre-barostat
+ * force evaluation (the gradients of its main and post-barostat evaluations are read by
+ * nothing before the next step's pre-barostat evaluation); after the call gpos, like every
+ * other array and scalar, holds exactly what nsteps calls of one step would leave. */
 int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream);
 
 /* Apply pending velocity/position transforms so that pos and vel hold the reference state. */

@@ -63,6 +63,7 @@ enum Phase {
   kPhRmsd = 10,          // rmsd_gpos / rmsd_delta
   kPhApplied = 11,       // pending transforms were applied
   kPhBaroEndPostPre = 12,  // kPhBaroEndPost, then the next step's kPhBaroPre (steps in a row)
+  kPhNvtEndPre = 13,       // kPhNvtEnd, then the next step's kPhNvtPre (steps in a row)
 };
 
 // ----------------------------------------------------------------------------- node kernels
@@ -772,12 +773,12 @@ __device__ void reduce_blocks(const double* __restrict__ part, int nb,
 __device__ __forceinline__ bool phase_uses_force(int phase) {
   return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
          phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
-         phase == kPhBaroEndPostPre;
+         phase == kPhBaroEndPostPre || phase == kPhNvtEndPre;
 }
 __device__ __forceinline__ bool phase_uses_nodes(int phase) {
   return phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
          phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
-         phase == kPhBaroEndPostPre || phase == kPhRmsd;
+         phase == kPhBaroEndPostPre || phase == kPhNvtEndPre || phase == kPhRmsd;
 }
 
 // The scalar update of one phase on the scalar block S of system sys.
@@ -985,10 +986,13 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
   for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sl[q] = Sg[q];
   __syncthreads();
   if (threadIdx.x == 0) {
+    // the end of one step and the start of the next, on the same copy of the block
     if (phase == kPhBaroEndPostPre) {
-      // the end of one step and the start of the next, on the same copy of the block
       run_phase(kPhBaroEndPost, Sl, f, nd, with_vtens, sys);
       run_phase(kPhBaroPre, Sl, f, nd, with_vtens, sys);
+    } else if (phase == kPhNvtEndPre) {
+      run_phase(kPhNvtEnd, Sl, f, nd, with_vtens, sys);
+      run_phase(kPhNvtPre, Sl, f, nd, with_vtens, sys);
     } else {
       run_phase(phase, Sl, f, nd, with_vtens, sys);
     }
@@ -1021,9 +1025,9 @@ struct mm_md {
   unsigned int* counters = nullptr;  // [nsys] last-block counters (kept at zero between calls)
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  // One step; barostat runs with several steps in a call replay, after the opening scalar
-  // phase, the lean step without it (which opens the next step) and then the full step
-  // without it.
+  // One step; runs whose step opens with a scalar phase replay, for several steps in a call,
+  // that phase, then the lean step without it (which opens the next step) and then the full
+  // step without it.
   hipGraph_t graph = nullptr, graph_lean = nullptr, graph_last = nullptr;
   hipGraphExec_t exec = nullptr, exec_lean = nullptr, exec_last = nullptr;
   int graph_split = 0;                // the field's block split the graphs were captured with
@@ -1095,7 +1099,8 @@ hipError_t node(mm_md* md, int mode, hipStream_t s) {
 bool reduces(int phase) {
   return phase == kPhInitForce || phase == kPhNveEnd || phase == kPhNvtEnd ||
          phase == kPhMainEndNpt || phase == kPhInitProps || phase == kPhBaroEndPre ||
-         phase == kPhBaroEndPost || phase == kPhBaroEndPostPre || phase == kPhRmsd;
+         phase == kPhBaroEndPost || phase == kPhBaroEndPostPre || phase == kPhNvtEndPre ||
+         phase == kPhRmsd;
 }
 
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
@@ -1155,15 +1160,19 @@ hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, int gmode = kGathe
   return hipSuccess;
 }
 
-// One step. lean (barostat runs): a step that another step follows before anything reads the
-// node gradients. The gradient of its main force evaluation is then used only by the kick in
-// the gather, and that of the barostat's post call by nothing (the next step's pre call
-// evaluates the force again before its kick, verlet.py:141-143 with npt.py:616-620), so those
-// evaluations keep their gradients out of HBM; positions, velocities, energies, virials and
-// every scalar are those of the full step, bit for bit.
-// Barostat runs also split a call's steps at their first scalar phase: with pre = false the
-// step leaves out its opening kPhBaroPre (the step before ran it), and a lean step ends with
-// kPhBaroEndPostPre, which opens the next step too.
+// Ensembles whose step opens with a scalar phase (the barostat's or the chain's pre update).
+bool opens_with_phase(const mm_md* md) {
+  return md->baro != 0 || md->thermo == MM_THERMO_NHC;
+}
+
+// One step. lean: a step that another step of the same call follows, so that nothing reads
+// the node gradients or the scalar block in between. With a barostat, the gradient of its main
+// force evaluation is then used only by the kick in the gather, and that of the barostat's post
+// call by nothing (the next step's pre call evaluates the force again before its kick,
+// verlet.py:141-143 with npt.py:616-620), so those evaluations keep their gradients out of
+// HBM; and a lean step ends with the next step's opening phase (kPhBaroEndPostPre,
+// kPhNvtEndPre), so that step is enqueued with pre = false. Positions, velocities, energies,
+// virials and every scalar are those of full steps, bit for bit.
 hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = true) {
   const bool lt = md->thermo == MM_THERMO_LANGEVIN;
   const int kick_drift = lt ? kKickDriftLT : kKickDrift;
@@ -1176,10 +1185,10 @@ hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = 
     MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s,
                      lean ? kGatherNone : kGatherStore));
   } else if (md->thermo == MM_THERMO_NHC) {
-    MM_TRY(scalar(md, kPhNvtPre, 1, s));
+    if (pre) MM_TRY(scalar(md, kPhNvtPre, 1, s));
     MM_TRY(node(md, kKickDrift, s));
     MM_TRY(force(md, true, false, s));
-    MM_TRY(scalar(md, kPhNvtEnd, 1, s));
+    MM_TRY(scalar(md, lean ? kPhNvtEndPre : kPhNvtEnd, 1, s));
   } else if (lt) {
     MM_TRY(node(md, kKickDriftLT, s));
     MM_TRY(force(md, true, true, s));
@@ -1355,14 +1364,14 @@ int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream) {
     if (md->graph_split != md->field->split) drop_graph(md);
     md->graph_split = md->field->split;
     // every step but the last is followed by another before the call returns
-    const bool lean = md->baro != 0 && nsteps > 1;
+    const bool lean = opens_with_phase(md) && nsteps > 1;
     if (!lean && !md->exec) e = capture_step(md, false, true, &md->graph, &md->exec);
     if (lean && !md->exec_lean) e = capture_step(md, true, false, &md->graph_lean, &md->exec_lean);
     if (lean && e == hipSuccess && !md->exec_last)
       e = capture_step(md, false, false, &md->graph_last, &md->exec_last);
     if (e != hipSuccess) return hip_error(e, "capturing the MD step");
     if (lean) {
-      e = scalar(md, kPhBaroPre, 1, md->stream);
+      e = scalar(md, md->baro ? kPhBaroPre : kPhNvtPre, 1, md->stream);
       for (int i = 0; i + 1 < nsteps && e == hipSuccess; ++i)
         e = hipGraphLaunch(md->exec_lean, md->stream);
       if (e == hipSuccess) e = hipGraphLaunch(md->exec_last, md->stream);
@@ -1370,7 +1379,7 @@ int mm_md_step(mm_md* md, int32_t nsteps, int use_graph, void* stream) {
       for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = hipGraphLaunch(md->exec, md->stream);
     }
   } else {
-    const bool lean = md->baro != 0 && nsteps > 1;
+    const bool lean = opens_with_phase(md) && nsteps > 1;
     for (int i = 0; i < nsteps && e == hipSuccess; ++i)
       e = enqueue_step(md, md->stream, lean && i + 1 < nsteps, !lean || i == 0);
   }

@@ -134,12 +134,13 @@ def test_graph_and_eager_agree_bitwise():
 
 
 @pytest.mark.parametrize("use_graph", [True, False])
-@pytest.mark.parametrize("ensemble", ["npt-nhc-mtk", "nph-mtk"])
+@pytest.mark.parametrize("ensemble", ["npt-nhc-mtk", "nph-mtk", "nvt-nhc"])
 def test_steps_of_one_call_match_single_steps_bitwise(ensemble, use_graph):
-    """Barostat runs: every step of an mm_md_step call but the last keeps the node gradients
-    of its main and post-barostat force evaluations out of HBM (nothing reads them before the
-    next step's pre-barostat evaluation). One 30-step call must equal 30 one-step calls (all
-    full steps) bit for bit: positions, velocities, the gradient after the run, every scalar."""
+    """Every step of an mm_md_step call but the last ends with the next step's opening scalar
+    phase, and with a barostat keeps the node gradients of its main and post-barostat force
+    evaluations out of HBM (nothing reads them before the next step's pre-barostat
+    evaluation). One 30-step call must equal 30 one-step calls (all full steps) bit for bit:
+    positions, velocities, the gradient after the run, every scalar."""
     print("lean steps", ensemble, use_graph, flush=True)
     _, v1, _ = build((4, 5, 3), 0.3, 0.05, 8, ensemble, "corner8")
     _, v2, _ = build((4, 5, 3), 0.3, 0.05, 8, ensemble, "corner8")
