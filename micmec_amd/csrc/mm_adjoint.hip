@@ -79,10 +79,32 @@ template <> __device__ __forceinline__ Dual make<Dual>(double v, double d) { ret
 template <typename T> struct IsDual { static constexpr bool value = false; };
 template <> struct IsDual<Dual> { static constexpr bool value = true; };
 
-__device__ __forceinline__ double xs(double v, int mask) { return __shfl_xor(v, mask, 64); }
-__device__ __forceinline__ Dual xs(Dual v, int mask) {
-  return {__shfl_xor(v.v, mask, 64), __shfl_xor(v.d, mask, 64)};
+// Lane exchanges inside groups of eight lanes, as DPP moves (VALU, no LDS round trip): lane l
+// receives the value of lane l ^ mask. Masks 1 and 2 are quad permutations ([1,0,3,2] and
+// [2,3,0,1]); mask 4 swaps the two 4-lane banks of each half row, as a row shift left by 4
+// written to banks 0 and 2 and a row shift right by 4 written to banks 1 and 3. A double
+// moves as its two 32-bit halves.
+template <int kCtrl, int kBanks>
+__device__ __forceinline__ double dpp64(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, kCtrl, 0xF, kBanks, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), kCtrl, 0xF, kBanks,
+                                             false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
 }
+template <int kMask>
+__device__ __forceinline__ double xs8(double v) {
+  static_assert(kMask == 1 || kMask == 2 || kMask == 4, "exchanges inside 8 lanes");
+  if (kMask == 1) return dpp64<0xB1, 0xF>(v, v);
+  if (kMask == 2) return dpp64<0x4E, 0xF>(v, v);
+  const double t = dpp64<0x104, 0x5>(v, v);
+  return dpp64<0x114, 0xA>(t, v);
+}
+__device__ __forceinline__ double xs(double v, int mask) {
+  return mask == 1 ? xs8<1>(v) : mask == 2 ? xs8<2>(v) : xs8<4>(v);
+}
+__device__ __forceinline__ Dual xs(Dual v, int mask) { return {xs(v.v, mask), xs(v.d, mask)}; }
 __device__ __forceinline__ double bc(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ Dual bc(Dual v, int src) {
   return {__shfl(v.v, src, 64), __shfl(v.d, src, 64)};
@@ -132,8 +154,8 @@ __device__ __forceinline__ void flush(const double* vsh, int nq, double* out, in
 #pragma unroll
     for (int c = 0; c < 8; ++c) s += vsh[(part * 8 + c) * 64 + col];
   }
-  s += __shfl_xor(s, 1, 64);
-  s += __shfl_xor(s, 2, 64);
+  s += xs8<1>(s);
+  s += xs8<2>(s);
   if (part == 0 && (q < nq || q == extra)) {
     int64_t r;
     if (q == extra) r = xbase + half;
@@ -406,8 +428,8 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
 #pragma unroll
       for (int c = 0; c < 8; ++c) sum += vsh[(pq * 8 + c) * 64 + col];
     }
-    sum += __shfl_xor(sum, 1, 64);
-    sum += __shfl_xor(sum, 2, 64);
+    sum += xs8<1>(sum);
+    sum += xs8<2>(sum);
     if (pq == 0 && q < kSysQ)
       part[(int64_t)(half == 0 ? q : kSysQ + q) * nblk + blockIdx.x] = sum;
     if (pq == 0 && q == kSysQ)
