@@ -118,6 +118,36 @@ __device__ __forceinline__ T sum8(T v) {
   return v;
 }
 
+// Sums over the 8 lanes of a cell of eight values c[0..7] at once: lane b returns the sum of
+// c[b]. Each level keeps half of the lane's values and adds the partner's copy of them, over
+// the same pairs, quads and halves as sum8, so every sum equals sum8 of that value bit for bit
+// (addition commutes), with 7 exchanges instead of 24.
+template <typename T>
+__device__ __forceinline__ T rs8(const T (&c)[8], int b) {
+  const bool b0 = (b & 1) != 0, b1 = (b & 2) != 0, b2 = (b & 4) != 0;
+  T a[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const T keep = b0 ? c[2 * k + 1] : c[2 * k];
+    const T send = b0 ? c[2 * k] : c[2 * k + 1];
+    a[k] = keep + xs(send, 1);  // value 2k + b0
+  }
+  T a2[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const T keep = b1 ? a[2 * m + 1] : a[2 * m];
+    const T send = b1 ? a[2 * m] : a[2 * m + 1];
+    a2[m] = keep + xs(send, 2);  // value 4m + 2 b1 + b0
+  }
+  const T keep = b2 ? a2[1] : a2[0];
+  const T send = b2 ? a2[0] : a2[1];
+  return keep + xs(send, 4);
+}
+
+// Upper triangle of the 6x6 Voigt matrix, row by row: entry q is (kUpperI[q], kUpperJ[q]).
+__device__ constexpr int kUpperI[21] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5};
+__device__ constexpr int kUpperJ[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
+
 constexpr int kAdjCells = 32;  // cells per block
 constexpr int kAdjPerSeg = kCellBlock / kAdjCells;  // adjoint blocks per cell-pass block
 static_assert(kCellBlock % kAdjCells == 0, "cell-pass blocks hold whole adjoint blocks");
@@ -355,26 +385,38 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
         const T G = (P[0 + i] * Tm[0 + j] + P[3 + i] * Tm[3 + j]) + P[6 + i] * Tm[6 + j];
         gacc[3 * i + j] = gacc[3 * i + j] + sw * G;
       }
-    // parameter entries of this state, summed over the cell's corners
-    int q = 0;
-#pragma unroll
-    for (int I = 0; I < 6; ++I)
-#pragma unroll
-      for (int J = I; J < 6; ++J) {
-        const T c = 0.5 * (e[I] * e[J]);
-        stage(vrow, q++, sw * (kModel == 0 ? sum8(c) : c), lead, valid);
+    // parameter entries of this state, summed over the cell's corners: entry q < 21 is
+    // 1/2 e_I e_J (upper triangle), 21 <= q < 30 is (dU/dB X^T)_ik = (dU/dP)_ik, 30 is u
+    auto entry = [&](int q) -> T {
+      if (q < 21) return 0.5 * (e[kUpperI[q]] * e[kUpperJ[q]]);
+      if (q < 30) {
+        const int i = (q - 21) / 3, k = (q - 21) % 3;
+        return (Tm[3 * i + 0] * X[3 * k + 0] + Tm[3 * i + 1] * X[3 * k + 1]) +
+               Tm[3 * i + 2] * X[3 * k + 2];
       }
+      return q == 30 ? u : make<T>(0.0, 0.0);
+    };
+    if (kModel == 0) {
+      // eight entries at a time, summed over the cell's lanes by a reduce-scatter: lane b ends
+      // with entry 8g + b and stages it (entry 31, the weight p_s, is not a sum)
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int g = 0; g < kNps / 8; ++g) {
+        T c[8];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        // dU/dP = (dU/dB) X^T
-        const T c = (Tm[3 * i + 0] * X[3 * k + 0] + Tm[3 * i + 1] * X[3 * k + 1]) +
-                    Tm[3 * i + 2] * X[3 * k + 2];
-        stage(vrow, q++, sw * (kModel == 0 ? sum8(c) : c), lead, valid);
+        for (int i = 0; i < 8; ++i) c[i] = entry(8 * g + i);
+        const T sum = rs8(c, b);
+        const int q = 8 * g + b;
+        const T v = q == 31 ? ps : (q == 30 ? ps : sw) * sum;
+        vrow[q] = valid ? val(v) : 0.0;
+        if (IsDual<T>::value) vrow[32 + q] = valid ? tan_(v) : 0.0;
       }
-    stage(vrow, 30, ps * (kModel == 0 ? sum8(u) : u), lead, valid);
-    stage(vrow, 31, ps, lead, valid);
+    } else {
+      // the averaged model's entries are the same on all eight lanes of a cell
+#pragma unroll
+      for (int q = 0; q < 30; ++q) stage(vrow, q, sw * entry(q), lead, valid);
+      stage(vrow, 30, ps * u, lead, valid);
+      stage(vrow, 31, ps, lead, valid);
+    }
     flush<T>(vsh, kNps, ppart, (int64_t)2 * s * kNps, 2, (int64_t)2 * s * kNps + 1, 2, nblk);
   }
 
@@ -403,20 +445,31 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
   }
   // energy, virial sum_v g_v (x) (r_v - r_0) and the domain-vector gradient: corner b moves
   // with domain vector d when the cell wraps along d and the corner lies at +d
-  stage(vrow, 0, ecell, lead, valid);
+  // (entries 1..18 summed over the cell's lanes eight at a time, as the parameter entries;
+  // the energy and dE/dkT, the same on every lane of the cell, ride along as entries 0 and
+  // kSysQ, the latter into its parameter rows)
+  auto sys_entry = [&](int q) -> T {
+    if (q >= 1 && q <= 9) return g[(q - 1) / 3] * (r[(q - 1) % 3] - r0[(q - 1) % 3]);
+    if (q >= 10 && q <= 18) {
+      const int d = (q - 10) / 3;
+      const double moves = (double)(((b & wrap) >> d) & 1);
+      return moves * g[(q - 10) % 3];
+    }
+    return make<T>(0.0, 0.0);
+  };
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
+  for (int gq = 0; gq < 3; ++gq) {
+    T c[8];
 #pragma unroll
-    for (int l = 0; l < 3; ++l)
-      stage(vrow, 1 + 3 * k + l, sum8(g[k] * (r[l] - r0[l])), lead, valid);
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double moves = (double)(((b & wrap) >> d) & 1);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) stage(vrow, 10 + 3 * d + j, sum8(moves * g[j]), lead, valid);
+    for (int i = 0; i < 8; ++i) c[i] = sys_entry(8 * gq + i);
+    const T sum = rs8(c, b);
+    const int q = 8 * gq + b;
+    const T v = q == 0 ? ecell : q == kSysQ ? dkt : sum;
+    if (q <= kSysQ) {
+      vrow[q] = valid ? val(v) : 0.0;
+      if (IsDual<T>::value) vrow[32 + q] = valid ? tan_(v) : 0.0;
+    }
   }
-  // dE/dkT rides along in column kSysQ of the same flush, into its parameter rows
-  stage(vrow, kSysQ, dkt, lead, valid);
   __syncthreads();
   {
     // the system rows go to `part`, the kT entry to `ppart`: two writers of one staged block
