@@ -279,9 +279,10 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
     for (int i = 0; i < 9; ++i) X[i] = 0.125 * sum8(X[i]);
   }
 
-  // pass 1: E_s = w * sum_a u_a + f_s for every state (mmff.py:380-384)
+  // pass 1 (multistate rows): E_s = w * sum_a u_a + f_s for every state (mmff.py:380-384),
+  // which the weights of pass 2 need first; a single-state cell's energy comes out of pass 2
   double emin = 0.0;
-  for (int s = 0; s < ns; ++s) {
+  for (int s = 0; s < (ns > 1 ? ns : 0); ++s) {
     const double* ts = tab + s * kTabStride;
     const double* P = ts;
     const double* Cv = ts + kCvOffset;
@@ -314,12 +315,10 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
     }
     emin = (s == 0 || val(es) < emin) ? val(es) : emin;
   }
-  __syncthreads();
+  if (ns > 1) __syncthreads();  // uniform over the block (one row per block)
   // E = E_min - kT ln Z, Z = sum_s exp(-(E_s - E_min)/kT) (mmff.py:386-395); one state: E = E_0
-  T ecell, zsum = make<T>(1.0, 0.0), dkt = make<T>(0.0, 0.0);
-  if (ns == 1) {
-    ecell = make<T>(es_sh[cl * 2], es_sh[cl * 2 + 1]);
-  } else {
+  T ecell = make<T>(0.0, 0.0), zsum = make<T>(1.0, 0.0), dkt = make<T>(0.0, 0.0);
+  if (ns > 1) {
     zsum = make<T>(0.0, 0.0);
     for (int s = 0; s < ns; ++s) {
       const T es = make<T>(es_sh[(s * kAdjCells + cl) * 2], es_sh[(s * kAdjCells + cl) * 2 + 1]);
@@ -369,6 +368,7 @@ adjoint_kernel(const double* __restrict__ pos, const double* __restrict__ dpos,
       u = u + e[I] * sv[I];
     }
     u = 0.5 * u;
+    if (ns == 1) ecell = wt * (kModel == 0 ? sum8(u) : u) + ts[10];
     // dU/dB = S B with S the symmetric stress of the Voigt vector Cv e; dU/dX = P^T (S B)
     const T S[9] = {sv[0], sv[5], sv[4], sv[5], sv[1], sv[3], sv[4], sv[3], sv[2]};
     T Tm[9];
