@@ -735,6 +735,28 @@ __device__ void take_nodes(double* S, const double (&nd)[kNodeSums]) {
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
 }
 
+// The node sums after the pending barostat matrix M, from those before it: the barostat call
+// changes every velocity to v M (row vectors, npt.py:709-726), so sum m (v M) (x) (v M) =
+// M^T (sum m v (x) v) M and the kinetic energy is half its trace. Only the NaN flag comes from
+// the node pass (the gather of the same force).
+__device__ void take_nodes_after_vmat(double* S, const double (&nd)[kNodeSums]) {
+#pragma clang fp contract(off)
+  if (S[MM_S_VMAT_ON] != 0.0) {
+    const double* M = S + MM_S_VMAT;
+    const double* V = S + MM_S_SVV;
+    double T[9], R[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        T[3 * i + j] = (V[3 * i + 0] * M[0 + j] + V[3 * i + 1] * M[3 + j]) + V[3 * i + 2] * M[6 + j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        R[3 * i + j] = (M[0 + i] * T[0 + j] + M[3 + i] * T[3 + j]) + M[6 + i] * T[6 + j];
+    for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] = R[q];
+    S[MM_S_EKIN] = 0.5 * ((R[0] + R[4]) + R[8]);
+  }
+  if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
+}
+
 // Sum SoA partials part[q * nb + b] over the blocks b = idx[first + i] (or first + i),
 // i < count, in a fixed order; every thread of the block receives the totals.
 template <int NV>
@@ -845,14 +867,15 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       }
       baro_call_begin(S);
       baro_part1(S, sys, 0);
-      // the velocity transform of this call does not depend on its force: it is applied by
-      // the gather that follows the force
+      // the velocity transform of this call does not depend on its force; it stays pending
+      // for the kick-drift pass, which applies it before the chain's factor
       baro_part2(S);
       break;
     case kPhBaroEndPre: {
-      clear_vtransforms(S);
+      // the barostat's velocity transform is still pending (see kPhBaroPre): the node sums
+      // after it follow from those before it
       take_force(S, f, true);
-      take_nodes(S, nd);
+      take_nodes_after_vmat(S, nd);
       update_baro_vel(S, sys, 0, 1);
       baro_call_end(S);
       if (baro_chain_on(S)) baro_chain_step(S);
@@ -1152,10 +1175,12 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
   } while (0)
 
 // The barostat call MTKBarostat.baro after its first scalar phase: positions scaled, force,
-// velocities transformed in the gather, then the second barostat-velocity update.
-hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, int gmode = kGatherStore) {
+// then the second barostat-velocity update. The pre call's velocity transform waits for the
+// kick-drift pass (scale_v = false); the post call's is applied by the gather (scale_v).
+hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, bool scale_v,
+                     int gmode = kGatherStore) {
   MM_TRY(node(md, kScalePos, s));
-  MM_TRY(force(md, false, false, s, true, gmode));
+  MM_TRY(force(md, false, false, s, scale_v, gmode));
   MM_TRY(scalar(md, end_phase, 1, s));
   return hipSuccess;
 }
@@ -1178,11 +1203,11 @@ hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = 
   const int kick_drift = lt ? kKickDriftLT : kKickDrift;
   if (md->baro) {
     if (pre) MM_TRY(scalar(md, kPhBaroPre, 1, s));
-    MM_TRY(baro_call(md, kPhBaroEndPre, s));
+    MM_TRY(baro_call(md, kPhBaroEndPre, s, false));
     MM_TRY(node(md, kick_drift, s));
     MM_TRY(force(md, true, lt, s, false, lean ? kGatherKeep : kGatherStore));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
-    MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s,
+    MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s, true,
                      lean ? kGatherNone : kGatherStore));
   } else if (md->thermo == MM_THERMO_NHC) {
     if (pre) MM_TRY(scalar(md, kPhNvtPre, 1, s));
