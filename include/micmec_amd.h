@@ -292,6 +292,10 @@ void mm_adjoint_destroy(mm_adjoint* a);
 #define MM_S_BCHAIN_LEN 146
 #define MM_S_BCHAIN_TEMP 147
 #define MM_S_BCHAIN_NDOF 148
+/* A second pending velocity matrix, applied after VMAT: the next barostat call's transform
+ * while the previous call's still waits (steps of one mm_md_step call). */
+#define MM_S_VMAT2 149    /* 9 */
+#define MM_S_VMAT2_ON 158
 #define MM_S_BCHAIN_MASS 160 /* 8 */
 #define MM_S_BCHAIN_POS 168  /* 8 */
 #define MM_S_BCHAIN_VEL 176  /* 8 */

@@ -147,7 +147,8 @@ __device__ __forceinline__ void vecmat(const double v[3], const double* M, doubl
 }
 
 // Pending velocity transforms, applied in the order the reference performs them:
-// factor A (thermostat post of the previous half), matrix M (barostat), factor B (thermostat pre).
+// factor A (thermostat post of the previous half), matrix M (barostat), matrix M2 (the next
+// barostat call, when M still waits), factor B (thermostat pre).
 __device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, double v[3]) {
 #pragma clang fp contract(off)
   if (S[MM_S_VSCALE_A_ON] != 0.0) {
@@ -158,6 +159,12 @@ __device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, d
   if (S[MM_S_VMAT_ON] != 0.0) {
     double t[3];
     vecmat(v, S + MM_S_VMAT, t);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = t[j];
+  }
+  if (S[MM_S_VMAT2_ON] != 0.0) {
+    double t[3];
+    vecmat(v, S + MM_S_VMAT2, t);
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = t[j];
   }
@@ -602,10 +609,13 @@ __device__ void baro_part1(double* S, int sys, int post) {
 
 // Velocity transform of the barostat call: exp(-(v_g + tr(v_g)/ndof) dt / 2) (npt.py:709-726;
 // without the trace term under a volume constraint, and exp(-(1 + 3/ndof) v_g dt / 2) in the
-// isotropic form, npt.py:466-484), composed after any pending velocity factor.
+// isotropic form, npt.py:466-484), pending after any pending velocity factor and matrix (into
+// the second matrix slot when the previous call's matrix still waits).
 __device__ void baro_part2(double* S) {
 #pragma clang fp contract(off)
   const double* vp = S + MM_S_VEL_PRESS;
+  const bool second = S[MM_S_VMAT_ON] != 0.0;
+  double* dst = S + (second ? MM_S_VMAT2 : MM_S_VMAT);
   if (baro_aniso(S)) {
     double A[9];
     for (int i = 0; i < 9; ++i) A[i] = vp[i];
@@ -616,12 +626,12 @@ __device__ void baro_part2(double* S) {
       A[4] += sh;
       A[8] += sh;
     }
-    expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], S + MM_S_VMAT);
+    expm_sym_half(A, -1.0, S[MM_S_TIMESTEP], dst);
   } else {
     const double f = exp(((-((1.0 + 3.0 / S[MM_S_NDOF]) * vp[0])) * S[MM_S_TIMESTEP]) / 2.0);
-    for (int i = 0; i < 9; ++i) S[MM_S_VMAT + i] = (i % 4 == 0) ? f : 0.0;
+    for (int i = 0; i < 9; ++i) dst[i] = (i % 4 == 0) ? f : 0.0;
   }
-  S[MM_S_VMAT_ON] = 1.0;
+  S[second ? MM_S_VMAT2_ON : MM_S_VMAT_ON] = 1.0;
 }
 
 // LangevinBarostat.pre/post bookkeeping (npt.py:374-392): energies at the start of the call.
@@ -651,6 +661,10 @@ __device__ void clear_vtransforms(double* S) {
   S[MM_S_VSCALE_A_ON] = 0.0;
   S[MM_S_VMAT_ON] = 0.0;
   S[MM_S_VSCALE_B_ON] = 0.0;
+  // the second matrix slot is used only by steps in a row; clearing its entries as well keeps
+  // the scalar block the same as after steps taken one per call
+  S[MM_S_VMAT2_ON] = 0.0;
+  for (int i = 0; i < 9; ++i) S[MM_S_VMAT2 + i] = 0.0;
 }
 
 __device__ void set_vscale(double* S, int slot, double f) {
@@ -735,14 +749,15 @@ __device__ void take_nodes(double* S, const double (&nd)[kNodeSums]) {
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
 }
 
-// The node sums after the pending barostat matrix M, from those before it: the barostat call
-// changes every velocity to v M (row vectors, npt.py:709-726), so sum m (v M) (x) (v M) =
+// The node sums after the last pending barostat matrix M, from those before it: the barostat
+// call changes every velocity to v M (row vectors, npt.py:709-726), so sum m (v M) (x) (v M) =
 // M^T (sum m v (x) v) M and the kinetic energy is half its trace. Only the NaN flag comes from
-// the node pass (the gather of the same force).
+// the node pass (the gather of the same force), when there is one.
 __device__ void take_nodes_after_vmat(double* S, const double (&nd)[kNodeSums]) {
 #pragma clang fp contract(off)
-  if (S[MM_S_VMAT_ON] != 0.0) {
-    const double* M = S + MM_S_VMAT;
+  const bool second = S[MM_S_VMAT2_ON] != 0.0;
+  if (second || S[MM_S_VMAT_ON] != 0.0) {
+    const double* M = S + (second ? MM_S_VMAT2 : MM_S_VMAT);
     const double* V = S + MM_S_SVV;
     double T[9], R[9];
     for (int i = 0; i < 3; ++i)
@@ -800,12 +815,15 @@ __device__ __forceinline__ bool phase_uses_force(int phase) {
 __device__ __forceinline__ bool phase_uses_nodes(int phase) {
   return phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
          phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
-         phase == kPhBaroEndPostPre || phase == kPhNvtEndPre || phase == kPhRmsd;
+         phase == kPhNvtEndPre || phase == kPhRmsd;
 }
 
-// The scalar update of one phase on the scalar block S of system sys.
+// The scalar update of one phase on the scalar block S of system sys. chained: the phase closes
+// a step that another step of the same call follows (kPhBaroEndPost: its velocity transforms
+// stay pending for the next kick-drift pass).
 __device__ void run_phase(int phase, double* S, const double (&f)[10],
-                          const double (&nd)[kNodeSums], int with_vtens, int sys) {
+                          const double (&nd)[kNodeSums], int with_vtens, int sys,
+                          bool chained = false) {
   const bool nhc = thermo_nhc(S);
   const bool lt = S[MM_S_THERMO] == (double)MM_THERMO_LANGEVIN;
   // the barostat's pressure contribution enters the chain only for MTK (npt.py:109-113)
@@ -860,11 +878,8 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
         S[MM_S_HAS_CHAINVEL0] = 1.0;
         S[MM_S_CHAINVEL0] = S[MM_S_BCHAIN_VEL];
       }
-      // a pending thermostat factor scales m v (x) v by its square
-      if (S[MM_S_VSCALE_A_ON] != 0.0) {
-        const double fac = S[MM_S_VSCALE_A];
-        for (int q = 0; q < 9; ++q) S[MM_S_SVV + q] *= fac * fac;
-      }
+      // (a pending thermostat factor is already in m v (x) v: the phases that set one scale
+      // the sums by its square)
       baro_call_begin(S);
       baro_part1(S, sys, 0);
       // the velocity transform of this call does not depend on its force; it stays pending
@@ -920,9 +935,12 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       break;
     }
     case kPhBaroEndPost:
-      clear_vtransforms(S);
+      // the node sums after the call's velocity transform follow from those before it (as in
+      // kPhBaroEndPre), whether the gather applied it (a call's last step) or it stays
+      // pending (chained)
       take_force(S, f, true);
-      take_nodes(S, nd);
+      take_nodes_after_vmat(S, nd);
+      if (!chained) clear_vtransforms(S);
       update_baro_vel(S, sys, 1, 1);
       if (baro_langevin(S)) {
         baro_call_end(S);
@@ -1011,7 +1029,7 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
   if (threadIdx.x == 0) {
     // the end of one step and the start of the next, on the same copy of the block
     if (phase == kPhBaroEndPostPre) {
-      run_phase(kPhBaroEndPost, Sl, f, nd, with_vtens, sys);
+      run_phase(kPhBaroEndPost, Sl, f, nd, with_vtens, sys, true);
       run_phase(kPhBaroPre, Sl, f, nd, with_vtens, sys);
     } else if (phase == kPhNvtEndPre) {
       run_phase(kPhNvtEnd, Sl, f, nd, with_vtens, sys);
@@ -1138,8 +1156,8 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
 
 // force evaluation: cells, then gather (+ second half kick when kick, + the Langevin
 // thermostat's post update when lt; + the pending velocity transforms when scale_v). gmode
-// (gather_kick_kernel): kGatherNone (scale_v only) leaves out the records altogether, so the
-// cell pass skips them too.
+// (gather_kick_kernel): kGatherNone leaves out the records altogether, so the cell pass skips
+// them too, and without scale_v there is then no node pass at all (energy and virial only).
 hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = false,
                  int gmode = kGatherStore) {
   ForceArgs a = force_args(md);
@@ -1149,7 +1167,8 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
   }
   hipError_t e = launch_cells(a, s);
   if (e != hipSuccess) return e;
-  if (md->N > 0) {
+  // with no records, no transform to apply and no kick, the evaluation has no node pass
+  if (md->N > 0 && !(gmode == kGatherNone && !scale_v && !kick)) {
 #define MM_GATHER(K, T, V)                                                                   \
   hipLaunchKernelGGL((gather_kick_kernel<K, T, V>), dim3(md->nbn), dim3(kNodeBlock), 0, s,    \
                      a.ncell, a.gcell, md->N, a.rstride, md->gpos, md->vel, md->masses, md->S,      \
@@ -1207,7 +1226,9 @@ hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = 
     MM_TRY(node(md, kick_drift, s));
     MM_TRY(force(md, true, lt, s, false, lean ? kGatherKeep : kGatherStore));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
-    MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s, true,
+    // the post call's velocity transform stays pending in a lean step (its node sums follow
+    // from the ones before it), so its force evaluation needs no node pass
+    MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s, !lean,
                      lean ? kGatherNone : kGatherStore));
   } else if (md->thermo == MM_THERMO_NHC) {
     if (pre) MM_TRY(scalar(md, kPhNvtPre, 1, s));
