@@ -64,9 +64,13 @@ __device__ constexpr int kEdge[8][3] = {{0, 4, 8},  {0, 5, 9},  {1, 4, 10}, {2, 
 // Strain energy density of one corner (or of the averaged matrix) and, if kGrad, the
 // derivative with respect to the corner matrix M, both for one (type, state) entry `tab`.
 // M is given row-major (rows = edges). Returns u = e.Cv.e / 2; writes G = dU/dM (unscaled).
-template <bool kGrad>
+// kVir: also adds the corner's virial G^T M = (S B)^T B = B^T S B (P M = B; symmetric) to
+// V[6] = (00, 11, 22, 12, 02, 01) — the sum over its edges of dU/d(edge) (x) edge, which is what
+// the vertex gradients give as sum_v g_v (x) r_v — without forming G when kGrad is false.
+template <bool kGrad, bool kVir = false>
 __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, const double M[9],
-                                                double G[9], int cv_offset = kCvOffset) {
+                                                double G[9], int cv_offset = kCvOffset,
+                                                double* V = nullptr) {
   const double* P = tab;
   double B[9];
 #pragma unroll
@@ -93,7 +97,7 @@ __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, 
   }
   const double u = 0.5 * (e[0] * s[0] + e[1] * s[1] + e[2] * s[2] + e[3] * s[3] + e[4] * s[4] +
                           e[5] * s[5]);
-  if (kGrad) {
+  if (kGrad || kVir) {
     // S (symmetric): S00=s0 S11=s1 S22=s2 S12=s3 S02=s4 S01=s5 ; T = S B ; G = P^T T
     const double S[9] = {s[0], s[5], s[4], s[5], s[1], s[3], s[4], s[3], s[2]};
     double T[9];
@@ -104,12 +108,21 @@ __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, 
         T[3 * i + j] = S[3 * i + 0] * B[0 + j] + S[3 * i + 1] * B[3 + j] + S[3 * i + 2] * B[6 + j];
       }
     }
+    if (kGrad) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < 3; ++i) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        G[3 * i + j] = P[0 + i] * T[0 + j] + P[3 + i] * T[3 + j] + P[6 + i] * T[6 + j];
+        for (int j = 0; j < 3; ++j) {
+          G[3 * i + j] = P[0 + i] * T[0 + j] + P[3 + i] * T[3 + j] + P[6 + i] * T[6 + j];
+        }
       }
+    }
+    if (kVir) {
+      // (T^T B)_kl = sum_i T_ik B_il
+      constexpr int kK[6] = {0, 1, 2, 1, 0, 0}, kL[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        V[q] += T[0 + kK[q]] * B[0 + kL[q]] + T[3 + kK[q]] * B[3 + kL[q]] + T[6 + kK[q]] * B[6 + kL[q]];
     }
   }
   return u;
@@ -120,9 +133,13 @@ __device__ __forceinline__ double corner_energy(const double* __restrict__ tab, 
 // The weight w enters through the weighted Voigt matrix w Cv, so the corner terms come out
 // already multiplied by w. kUnit: gscale is 1 (single-state cells) and is not applied.
 // kModel: 0 = corner8 (nanocell.py), 1 = original (nanocell_original.py).
-template <int kModel, bool kGrad, bool kUnit = false>
+// kVir: also adds the state's virial sum_v g_v (x) r_v, as the sum of its corners' B^T S B
+// (see corner_energy; for the averaged matrix H the vertex rows are +-1/4 of dU/dH and
+// sum_v +-r_v along d is 4 H_d, so the same form), to V[6] — unscaled by gscale.
+template <int kModel, bool kGrad, bool kUnit = false, bool kVir = false>
 __device__ __forceinline__ double state_energy(const double* __restrict__ tab, const double r[8][3],
-                                               double g[8][3], double gscale = 1.0) {
+                                               double g[8][3], double gscale = 1.0,
+                                               double* V = nullptr) {
   if (kModel == 0) {
     // the twelve edges once (each is a row of two corner matrices)
     double E[12][3];
@@ -139,7 +156,7 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
 #pragma unroll
         for (int j = 0; j < 3; ++j) M[3 * d + j] = E[kEdge[a][d]][j];
       }
-      usum += corner_energy<kGrad>(tab, M, G, kCvwOffset);
+      usum += corner_energy<kGrad, kVir>(tab, M, G, kCvwOffset, V);
       if (kGrad) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -166,7 +183,7 @@ __device__ __forceinline__ double state_energy(const double* __restrict__ tab, c
         H[3 * d + j] = 0.125 * acc;
       }
     }
-    const double u = corner_energy<kGrad>(tab, H, G, kCvwOffset);
+    const double u = corner_energy<kGrad, kVir>(tab, H, G, kCvwOffset, V);
     if (kGrad) {
       const double q = kUnit ? 0.25 : 0.25 * gscale;
 #pragma unroll

@@ -108,23 +108,24 @@ __device__ constexpr int kUpperCorner[4] = {3, 5, 6, 7};
 
 // One lane per cell. Energies of all states of the block's cell type, combined by the
 // log-sum-exp of mmff.py:386-398; the vertex gradient of the combined energy goes to gcell and
-// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials (with store = 0
-// the gradient feeds only the virial: no face fold and no records). A block holds
+// the cell's energy and virial sum_v g_v (x) (r_v - r_0) enter the block partials. A block holds
 // one cell type; single-state and multistate blocks run in separate instances (kMulti), each
 // launched over its own range of blocks (blk0 + blockIdx.x) when the work order allows, so that
 // the single-state instance does not carry the registers of the state combination.
-template <int kModel, bool kGrad, bool kMulti>
-__global__ void __launch_bounds__(kCellBlock, 2)
+// kVir: the virial as the sum of the corners' B^T S B (state_energy), which the vertex
+// gradients give as sum_v g_v (x) r_v; with kGrad false the pass then forms no gradient at all
+// (energy and virial only).
+template <int kModel, bool kGrad, bool kMulti, bool kVir>
+__global__ void __launch_bounds__(kCellBlock, (kVir && !kGrad && !kMulti) ? 3 : 2)
 cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             int64_t rvecs_stride, const int32_t* __restrict__ blk_sys,
             const int32_t* __restrict__ cnode, const int32_t* __restrict__ cwrap,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
             int32_t max_states, int64_t W, const int32_t* __restrict__ cbase, int64_t S,
-            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks,
-            int store) {
+            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks) {
   __shared__ double red[10 * (kCellBlock / 64)];
-  __shared__ double es_sh[(kMulti && !kGrad) ? kMaxStates * kCellBlock : 1];
+  __shared__ double es_sh[(kMulti && !kGrad && !kVir) ? kMaxStates * kCellBlock : 1];
   // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
   __shared__ double gs_sh[(kMulti && kGrad) ? 24 * kCellBlock : 1];
   const int bid = blk0 + xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -154,6 +155,9 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   for (int v = 0; v < 8; ++v)
 #pragma unroll
     for (int j = 0; j < 3; ++j) gacc[v][j] = -0.0;
+  double vir[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) vir[q] = -0.0;
   const bool valid = n[0] >= 0;
   const int wrap = valid ? cwrap[w] : 0;
   if (valid) {
@@ -161,8 +165,8 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
     gather_vertices(pos, rv, n, wrap, r);
     double ecell;
     if (!kMulti) {
-      ecell = tab[10] + state_energy<kModel, kGrad, true>(tab, r, gacc);
-    } else if (!kGrad) {
+      ecell = tab[10] + state_energy<kModel, kGrad, true, kVir>(tab, r, gacc, 1.0, vir);
+    } else if (!kGrad && !kVir) {
       // energies of all states, E = E_min - kT ln sum exp(-(E_s - E_min)/kT) (mmff.py:386-395);
       // the state energies are kept in LDS
       double emin = 0.0;
@@ -179,42 +183,68 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       // one pass over the states: energy and gradient of each, and the gradient
       // sum_s p_s dE_s/dr with p_s = exp(-(E_s - E_min)/kT) / Z (mmff.py:386-398) accumulated
       // in LDS with weights relative to the lowest energy so far, rescaled when it moves
+      // (kVir: the state virials combine the same way, in registers)
       double* gsum = gs_sh + threadIdx.x;
-      double emin = 0.0, zsum = 0.0;
+      double emin = 0.0, zsum = 0.0, vsum[6];
       for (int s = 0; s < ns; ++s) {
         const double* ts = tab + s * kTabStride;
-        double gs[8][3];
+        double gs[8][3], vs[6];
 #pragma unroll
         for (int v = 0; v < 8; ++v)
 #pragma unroll
           for (int j = 0; j < 3; ++j) gs[v][j] = -0.0;
-        const double es = ts[10] + state_energy<kModel, true, true>(ts, r, gs);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) vs[q] = -0.0;
+        const double es = ts[10] + state_energy<kModel, kGrad, true, kVir>(ts, r, gs, 1.0, vs);
         if (s == 0) {
           emin = es;
           zsum = 1.0;
+          if (kGrad) {
 #pragma unroll
-          for (int q = 0; q < 24; ++q) gsum[q * kCellBlock] = gs[q / 3][q % 3];
+            for (int q = 0; q < 24; ++q) gsum[q * kCellBlock] = gs[q / 3][q % 3];
+          }
+#pragma unroll
+          for (int q = 0; q < 6; ++q) vsum[q] = vs[q];
         } else if (es < emin) {
           const double c = exp(-(emin - es) / kt);
           zsum = zsum * c + 1.0;
+          if (kGrad) {
 #pragma unroll
-          for (int q = 0; q < 24; ++q)
-            gsum[q * kCellBlock] = gsum[q * kCellBlock] * c + gs[q / 3][q % 3];
+            for (int q = 0; q < 24; ++q)
+              gsum[q * kCellBlock] = gsum[q * kCellBlock] * c + gs[q / 3][q % 3];
+          }
+#pragma unroll
+          for (int q = 0; q < 6; ++q) vsum[q] = vsum[q] * c + vs[q];
           emin = es;
         } else {
           const double c = exp(-(es - emin) / kt);
           zsum += c;
+          if (kGrad) {
 #pragma unroll
-          for (int q = 0; q < 24; ++q) gsum[q * kCellBlock] += c * gs[q / 3][q % 3];
+            for (int q = 0; q < 24; ++q) gsum[q * kCellBlock] += c * gs[q / 3][q % 3];
+          }
+#pragma unroll
+          for (int q = 0; q < 6; ++q) vsum[q] += c * vs[q];
         }
       }
       ecell = emin - kt * log(zsum);
       const double inv = 1.0 / zsum;
+      if (kGrad) {
 #pragma unroll
-      for (int q = 0; q < 24; ++q) gacc[q / 3][q % 3] = gsum[q * kCellBlock] * inv;
+        for (int q = 0; q < 24; ++q) gacc[q / 3][q % 3] = gsum[q * kCellBlock] * inv;
+      }
+      if (kVir) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) vir[q] = vsum[q] * inv;
+      }
     }
     acc[0] = ecell;
-    if (kGrad) {
+    if (kVir) {
+      // vtens[k][l] from the symmetric corner sums (00, 11, 22, 12, 02, 01)
+      constexpr int kSym[9] = {0, 5, 4, 5, 1, 3, 4, 3, 2};
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[1 + q] = vir[kSym[q]];
+    } else if (kGrad) {
       // virial: vtens[k][l] = sum_v g_v[k] (r_v - r_0)[l]
 #pragma unroll
       for (int v = 1; v < 8; ++v) {
@@ -225,7 +255,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
       }
     }
   }
-  if (kGrad && store) {
+  if (kGrad) {
     // The upper face (corners 3, 5, 6, 7) of the slot below is added to this slot's lower face
     // (corners 0, 1, 2, 4) when that slot holds the -z neighbour (wrap bit 3); the lower faces
     // go to their class planes, the upper faces that were not folded (wrap bit 4 clear) to the
@@ -423,10 +453,10 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
     if (e != hipSuccess) return e;
   }
 #define MM_LAUNCH_CELL(MODEL, GRAD, MULTI, S, B0, NBL)                                       \
-  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI>), dim3(NBL), dim3(kCellBlock), 0, S,   \
+  hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI, VIR>), dim3(NBL), dim3(kCellBlock), 0, S, \
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
                      a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
-                     a.partials, B0, nb, a.store_records)
+                     a.partials, B0, nb)
   // The single-state instance, which takes longer, is launched first, on the caller's stream:
   // it then starts right behind the preceding kernel instead of after the side stream's.
 #define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
@@ -434,11 +464,22 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
     if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, stream, s0, sn);                         \
     if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \
   } while (0)
-  if (a.model == 0) {
-    if (grad) MM_LAUNCH_PAIR(0, true); else MM_LAUNCH_PAIR(0, false);
+  // corner_virial: the virial from the corners; without records the pass then forms no
+  // gradient at all
+  const bool vir = a.corner_virial != 0;
+  const bool kgrad = grad && !(vir && !a.store_records);
+#define MM_LAUNCH_MODEL(MODEL)                                                                \
+  do {                                                                                        \
+    if (kgrad) MM_LAUNCH_PAIR(MODEL, true); else MM_LAUNCH_PAIR(MODEL, false);               \
+  } while (0)
+  if (vir) {
+    constexpr bool VIR = true;
+    if (a.model == 0) MM_LAUNCH_MODEL(0); else MM_LAUNCH_MODEL(1);
   } else {
-    if (grad) MM_LAUNCH_PAIR(1, true); else MM_LAUNCH_PAIR(1, false);
+    constexpr bool VIR = false;
+    if (a.model == 0) MM_LAUNCH_MODEL(0); else MM_LAUNCH_MODEL(1);
   }
+#undef MM_LAUNCH_MODEL
 #undef MM_LAUNCH_PAIR
 #undef MM_LAUNCH_CELL
   e = hipGetLastError();

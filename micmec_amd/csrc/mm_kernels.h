@@ -37,7 +37,9 @@ struct ForceArgs {
   int model;               // 0 corner8, 1 original
   int* nanflag;
   int force_grad = 0;       // compute gradients even when gpos and vtens are null
-  int store_records = 1;    // 0: the gradients feed only the virial (no face fold, no records)
+  int corner_virial = 0;    // 1: the virial as the corners' sum of B^T S B (no vertex gradient
+                            //    needed for it)
+  int store_records = 1;    // with corner_virial: 0 forms no gradient and stores no records
   int split = -1;           // multistate blocks are [0, split), single-state [split, nb); -1: mixed
   hipStream_t side = nullptr;   // stream of the multistate instance (null: the caller's stream)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

@@ -1159,8 +1159,9 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
 // (gather_kick_kernel): kGatherNone leaves out the records altogether, so the cell pass skips
 // them too, and without scale_v there is then no node pass at all (energy and virial only).
 hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = false,
-                 int gmode = kGatherStore) {
+                 int gmode = kGatherStore, bool corner_virial = false) {
   ForceArgs a = force_args(md);
+  a.corner_virial = corner_virial;
   if (gmode == kGatherNone) {
     if (kick) return hipErrorInvalidValue;
     a.store_records = 0;
@@ -1197,9 +1198,9 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
 // then the second barostat-velocity update. The pre call's velocity transform waits for the
 // kick-drift pass (scale_v = false); the post call's is applied by the gather (scale_v).
 hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, bool scale_v,
-                     int gmode = kGatherStore) {
+                     int gmode = kGatherStore, bool corner_virial = false) {
   MM_TRY(node(md, kScalePos, s));
-  MM_TRY(force(md, false, false, s, scale_v, gmode));
+  MM_TRY(force(md, false, false, s, scale_v, gmode, corner_virial));
   MM_TRY(scalar(md, end_phase, 1, s));
   return hipSuccess;
 }
@@ -1228,8 +1229,10 @@ hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = 
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
     // the post call's velocity transform stays pending in a lean step (its node sums follow
     // from the ones before it), so its force evaluation needs no node pass
+    // and its virial comes from the corners (B^T S B, no vertex gradient needed), in the
+    // last step too, so that lean and full steps agree to the bit
     MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s, !lean,
-                     lean ? kGatherNone : kGatherStore));
+                     lean ? kGatherNone : kGatherStore, true));
   } else if (md->thermo == MM_THERMO_NHC) {
     if (pre) MM_TRY(scalar(md, kPhNvtPre, 1, s));
     MM_TRY(node(md, kKickDrift, s));
