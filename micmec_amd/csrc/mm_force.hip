@@ -442,7 +442,7 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   const bool part = a.split >= 0;
   const int m0 = 0, mn = part ? a.split : nb;
   const int s0 = part ? a.split : 0, sn = part ? nb - a.split : nb;
-  // the multistate instance runs on the side stream, concurrently with the single-state one
+  // the second instance runs on the side stream, concurrently with the first
   const bool fork = a.side != nullptr && a.ev_fork != nullptr && a.ev_join != nullptr &&
                     mn > 0 && sn > 0;
   hipStream_t ms = fork ? a.side : stream;
@@ -457,17 +457,25 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
                      a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
                      a.partials, B0, nb)
-  // The single-state instance, which takes longer, is launched first, on the caller's stream:
-  // it then starts right behind the preceding kernel instead of after the side stream's.
-#define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
-  do {                                                                                       \
-    if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, stream, s0, sn);                         \
-    if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                              \
-  } while (0)
   // corner_virial: the virial from the corners; without records the pass then forms no
   // gradient at all
   const bool vir = a.corner_virial != 0;
   const bool kgrad = grad && !(vir && !a.store_records);
+  // The instance that takes longer is launched first, on the caller's stream: it then starts
+  // right behind the preceding kernel instead of after the side stream's. That is the
+  // single-state one, except without gradients and with the corner virial, where the
+  // single-state instance runs three waves a SIMD and the multistate one two.
+  const bool multi_lead = vir && !kgrad;
+#define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
+  do {                                                                                       \
+    if (multi_lead) {                                                                        \
+      if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, stream, m0, mn);                        \
+      if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, ms, s0, sn);                           \
+    } else {                                                                                 \
+      if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, stream, s0, sn);                       \
+      if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, ms, m0, mn);                            \
+    }                                                                                        \
+  } while (0)
 #define MM_LAUNCH_MODEL(MODEL)                                                                \
   do {                                                                                        \
     if (kgrad) MM_LAUNCH_PAIR(MODEL, true); else MM_LAUNCH_PAIR(MODEL, false);               \

@@ -252,14 +252,18 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // gmode: kGatherStore sums the records and stores gpos; kGatherKeep sums them for the kick only
 // (the step overwrites gpos before anything reads it); kGatherNone (kScaleV only) reads no
 // records: the force evaluation fed only the energy and the virial.
+// With kDrift instead, the first half kick and the drift of the step (verlet.py:141-146) with
+// the barostat's pre-call gradient, which nothing else reads: the pending velocity transforms
+// (and the Langevin thermostat's pre update, its ekin after the update in partial row 14),
+// then v += a dt / 2 and x += dt v, as the node pass kKickDrift does from a stored gpos.
 // One node per thread (the gathers need every wave in flight that they can get).
-template <bool kKick, bool kLT, bool kScaleV = false>
+template <bool kKick, bool kLT, bool kScaleV = false, bool kDrift = false>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t RS, double* __restrict__ gpos, double* __restrict__ v,
                    const double* __restrict__ m, const double* __restrict__ S0,
                    const int32_t* __restrict__ blk_sys, double* __restrict__ partials,
-                   int gmode) {
+                   int gmode, double* __restrict__ x = nullptr) {
   const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
   constexpr int NA = kLT ? 14 : 13;
   double acc[NA];
@@ -314,9 +318,34 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
       node_vv(vv, m[n], acc);
     }
+    if (kDrift) {
+#pragma clang fp contract(off)
+      const double dt = S[MM_S_TIMESTEP];
+      const double mass = m[n];
+      double vv[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
+      apply_vtransform(S, vv);
+      if (kLT) {
+        langevin_thermo(S, n, 0, mass, vv);
+        acc[0] += node_ekin(vv, mass);
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double a = -g[j] / mass;
+        vv[j] = vv[j] + (0.5 * a) * dt;
+        v[3 * n + j] = vv[j];
+        x[3 * n + j] = x[3 * n + j] + dt * vv[j];
+      }
+    }
     }
   }
-  if (kKick || kScaleV) {
+  if (kDrift) {
+    if (kLT) {
+      double lt[1] = {acc[0]};
+      block_sum_store_wide<1>(lt, partials + (int64_t)14 * gridDim.x);
+    }
+  } else if (kKick || kScaleV) {
     block_sum_store_wide<NA>(acc, partials);
   } else {
     // only the NaN flag: any lane's flag, per block
@@ -814,8 +843,8 @@ __device__ __forceinline__ bool phase_uses_force(int phase) {
 }
 __device__ __forceinline__ bool phase_uses_nodes(int phase) {
   return phase == kPhInitProps || phase == kPhNveEnd || phase == kPhNvtEnd ||
-         phase == kPhBaroEndPre || phase == kPhMainEndNpt || phase == kPhBaroEndPost ||
-         phase == kPhNvtEndPre || phase == kPhRmsd;
+         phase == kPhMainEndNpt || phase == kPhBaroEndPost || phase == kPhNvtEndPre ||
+         phase == kPhRmsd;
 }
 
 // The scalar update of one phase on the scalar block S of system sys. chained: the phase closes
@@ -888,7 +917,8 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       break;
     case kPhBaroEndPre: {
       // the barostat's velocity transform is still pending (see kPhBaroPre): the node sums
-      // after it follow from those before it
+      // after it follow from those before it. The gather of this force comes after the phase
+      // (it is the kick-drift pass), so a NaN gradient shows here through the virial.
       take_force(S, f, true);
       take_nodes_after_vmat(S, nd);
       update_baro_vel(S, sys, 0, 1);
@@ -1194,13 +1224,39 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
     if (_e != hipSuccess) return _e;      \
   } while (0)
 
+// The first half kick and the drift from the records of the last cell pass (the barostat's
+// pre call, whose gradient nothing else reads), after that call's scalar phase has set the
+// pending velocity transforms; with lt also the Langevin thermostat's pre update.
+hipError_t gather_drift(mm_md* md, bool lt, hipStream_t s) {
+  if (md->N == 0) return hipSuccess;
+  mm_field* f = md->field;
+#define MM_DRIFT(T)                                                                          \
+  hipLaunchKernelGGL((gather_kick_kernel<false, T, false, true>), dim3(md->nbn),             \
+                     dim3(kNodeBlock), 0, s, f->nrec, f->gcell, md->N, f->S, md->gpos,       \
+                     md->vel, md->masses, md->S, md->node_blk_sys, md->npart, kGatherKeep,   \
+                     md->pos)
+  if (lt)
+    MM_DRIFT(true);
+  else
+    MM_DRIFT(false);
+#undef MM_DRIFT
+  return hipGetLastError();
+}
+
 // The barostat call MTKBarostat.baro after its first scalar phase: positions scaled, force,
-// then the second barostat-velocity update. The pre call's velocity transform waits for the
-// kick-drift pass (scale_v = false); the post call's is applied by the gather (scale_v).
+// then the second barostat-velocity update. The pre call (pre_call) leaves its records for
+// gather_drift, which applies its velocity transform with the kick; the post call's transform
+// is applied by its gather (scale_v) or stays pending.
 hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, bool scale_v,
-                     int gmode = kGatherStore, bool corner_virial = false) {
+                     int gmode = kGatherStore, bool corner_virial = false,
+                     bool pre_call = false) {
   MM_TRY(node(md, kScalePos, s));
-  MM_TRY(force(md, false, false, s, scale_v, gmode, corner_virial));
+  if (pre_call) {
+    ForceArgs a = force_args(md);
+    MM_TRY(launch_cells(a, s));
+  } else {
+    MM_TRY(force(md, false, false, s, scale_v, gmode, corner_virial));
+  }
   MM_TRY(scalar(md, end_phase, 1, s));
   return hipSuccess;
 }
@@ -1220,11 +1276,11 @@ bool opens_with_phase(const mm_md* md) {
 // virials and every scalar are those of full steps, bit for bit.
 hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = true) {
   const bool lt = md->thermo == MM_THERMO_LANGEVIN;
-  const int kick_drift = lt ? kKickDriftLT : kKickDrift;
   if (md->baro) {
     if (pre) MM_TRY(scalar(md, kPhBaroPre, 1, s));
-    MM_TRY(baro_call(md, kPhBaroEndPre, s, false));
-    MM_TRY(node(md, kick_drift, s));
+    // the pre call's gradient drives only the first half kick: its gather is the kick-drift
+    MM_TRY(baro_call(md, kPhBaroEndPre, s, false, kGatherKeep, false, true));
+    MM_TRY(gather_drift(md, lt, s));
     MM_TRY(force(md, true, lt, s, false, lean ? kGatherKeep : kGatherStore));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
     // the post call's velocity transform stays pending in a lean step (its node sums follow
