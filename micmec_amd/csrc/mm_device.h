@@ -211,6 +211,17 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
   }
 }
 
+// Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
+// every XCD has its own L2. Neighbouring blocks share cache lines (the cell segments share
+// nodes, the node blocks' record gathers share the lines at their ends), so each XCD is given
+// one contiguous range of the n blocks: block b -> (b mod 8) * (n / 8) + min(b mod 8, n mod 8)
+// + b / 8. The map is a bijection of [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n / kXcds, r = n % kXcds, x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // One step of wave_reduce_scatter: the lane keeps the upper H of its 2H values when bit BIT of its
 // lane index is set, the lower H otherwise, and adds its partner's copy of them.
 template <int H, int BIT>

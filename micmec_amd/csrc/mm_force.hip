@@ -91,16 +91,6 @@ __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
   }
 }
 
-// Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
-// every XCD has its own L2. Neighbouring segments share nodes, so each XCD is given one
-// contiguous range of the n blocks: block b -> (b mod 8) * (n / 8) + min(b mod 8, n mod 8) + b / 8.
-// The map is a bijection of [0, n) for any n.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block(int b, int n) {
-  const int q = n / kXcds, r = n % kXcds, x = b % kXcds, i = b / kXcds;
-  return x * q + (x < r ? x : r) + i;
-}
-
 // Corners of the lower (z = 0) and upper (z = 1) face of a cell, in face-class order: upper
 // corner kUpperCorner[q] of a cell is lower corner kLowerCorner[q] of its +z neighbour.
 __device__ constexpr int kLowerCorner[4] = {0, 1, 2, 4};
@@ -341,7 +331,9 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
   __shared__ double sg[768];
   __shared__ double sd[DOT ? 768 : 1], so[DOT ? 768 : 1];
   const int t = threadIdx.x;
-  const int64_t n0 = (int64_t)blockIdx.x * 256, n = n0 + t;
+  // node blocks dealt to the XCDs in contiguous ranges: neighbouring blocks share record lines
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int64_t n0 = (int64_t)blk * 256, n = n0 + t;
   const int64_t e0 = 3 * n0;
   const int64_t ne = 3 * (N < n0 + 256 ? N - n0 : (int64_t)256);  // values of this workgroup
 #pragma unroll
@@ -393,7 +385,7 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
     const int li = r * 256 + t;
     if (li < ne) gpos[e0 + li] = sg[li];
   }
-  if (DOT) block_sum_max_store<3, 1>(acc, dpart + blockIdx.x, gridDim.x);
+  if (DOT) block_sum_max_store<3, 1>(acc, dpart + blk, gridDim.x);
 }
 
 // One block per item: item i reduces src[0..n) (sum in a fixed order, or a NaN-propagating

@@ -85,9 +85,10 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* part) {
   }
 }
 
-// The same for blocks of kNodeBlock threads (waves summed in order through LDS).
+// The same for blocks of kNodeBlock threads (waves summed in order through LDS), into the
+// partial slot of logical block blk.
 template <int NV>
-__device__ __forceinline__ void block_sum_store_wide(double (&v)[NV], double* part) {
+__device__ __forceinline__ void block_sum_store_wide(double (&v)[NV], double* part, int blk) {
   __shared__ double red[NV * (kNodeBlock / 64)];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   {
@@ -99,7 +100,7 @@ __device__ __forceinline__ void block_sum_store_wide(double (&v)[NV], double* pa
   if (threadIdx.x < NV) {
     double s = red[threadIdx.x];
     for (int w = 1; w < kNodeBlock / 64; ++w) s += red[w * NV + threadIdx.x];
-    part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    part[(int64_t)threadIdx.x * gridDim.x + blk] = s;
   }
 }
 
@@ -256,7 +257,9 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // the barostat's pre-call gradient, which nothing else reads: the pending velocity transforms
 // (and the Langevin thermostat's pre update, its ekin after the update in partial row 14),
 // then v += a dt / 2 and x += dt v, as the node pass kKickDrift does from a stored gpos.
-// One node per thread (the gathers need every wave in flight that they can get).
+// One node per thread (the gathers need every wave in flight that they can get); the node
+// blocks are dealt to the XCDs in contiguous ranges (xcd_block), and the partial sums are
+// stored per logical block.
 template <bool kKick, bool kLT, bool kScaleV = false, bool kDrift = false>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
@@ -264,13 +267,14 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
                    const double* __restrict__ m, const double* __restrict__ S0,
                    const int32_t* __restrict__ blk_sys, double* __restrict__ partials,
                    int gmode, double* __restrict__ x = nullptr) {
-  const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blockIdx.x] * MM_S_COUNT : 0);
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const double* S = S0 + (blk_sys ? (int64_t)blk_sys[blk] * MM_S_COUNT : 0);
   constexpr int NA = kLT ? 14 : 13;
   double acc[NA];
 #pragma unroll
   for (int q = 0; q < NA; ++q) acc[q] = 0.0;
   {
-    const int64_t n = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+    const int64_t n = (int64_t)blk * kNodeBlock + threadIdx.x;
     if (n < N) {
     double g[3] = {0.0, 0.0, 0.0};
     if (gmode != kGatherNone) {
@@ -343,10 +347,10 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
   if (kDrift) {
     if (kLT) {
       double lt[1] = {acc[0]};
-      block_sum_store_wide<1>(lt, partials + (int64_t)14 * gridDim.x);
+      block_sum_store_wide<1>(lt, partials + (int64_t)14 * gridDim.x, blk);
     }
   } else if (kKick || kScaleV) {
-    block_sum_store_wide<NA>(acc, partials);
+    block_sum_store_wide<NA>(acc, partials, blk);
   } else {
     // only the NaN flag: any lane's flag, per block
     const unsigned long long bad = __ballot(acc[12] != 0.0);
@@ -356,7 +360,7 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     if ((threadIdx.x & 63) == 0 && bad) any_bad = 1;
     __syncthreads();
     if (threadIdx.x < 13)
-      partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] =
+      partials[(int64_t)threadIdx.x * gridDim.x + blk] =
           (threadIdx.x == 12 && any_bad) ? 1.0 : 0.0;
   }
 }
@@ -536,13 +540,16 @@ __device__ void langevin_random_force(const double* S, int sys, int post, int su
   const uint64_t call = 2ull * (uint64_t)S[MM_S_COUNTER] + (uint64_t)post;
   const uint64_t key = seed_from(S + MM_S_LB_SEED);
   double z[8];
+#pragma unroll
   for (int b = 0; b < 2; ++b) {
     const uint64_t ctr[4] = {(uint64_t)sys, call, (uint64_t)sub, (uint64_t)b};
     normals4(ctr, key, kStreamBarostat, z + 4 * b);
   }
   const double sc = S[MM_S_LB_RSCALE];
   int k = 0;
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
+#pragma unroll
     for (int j = 0; j <= i; ++j) {
       const double r = z[k++] * sc;
       R[3 * i + j] = r;
@@ -554,7 +561,7 @@ __device__ void langevin_random_force(const double* S, int sys, int post, int su
 // update_baro_vel of MTKBarostat.baro (npt.py:654-681) and of LangevinBarostat.baro
 // (npt.py:394-432: friction exp(-dt / 8 tau) on both sides, random force R, optional volume
 // constraint and isotropic form). sub is 0 for the first and 1 for the second update of a call.
-__device__ void update_baro_vel(double* S, int sys, int post, int sub) {
+__device__ __forceinline__ void update_baro_vel(double* S, int sys, int post, int sub) {
 #pragma clang fp contract(off)
   const double dt = S[MM_S_TIMESTEP];
   double* vp = S + MM_S_VEL_PRESS;
@@ -621,7 +628,7 @@ __device__ void update_baro_vel(double* S, int sys, int post, int sub) {
 
 // First half of the barostat call: v_g update, then positions and domain scaled by
 // exp(v_g dt / 2) (npt.py:683-702, 434-452). The position transform is left in XMAT.
-__device__ void baro_part1(double* S, int sys, int post) {
+__device__ __forceinline__ void baro_part1(double* S, int sys, int post) {
 #pragma clang fp contract(off)
   update_baro_vel(S, sys, post, 0);
   double R[9];
@@ -640,7 +647,7 @@ __device__ void baro_part1(double* S, int sys, int post) {
 // without the trace term under a volume constraint, and exp(-(1 + 3/ndof) v_g dt / 2) in the
 // isotropic form, npt.py:466-484), pending after any pending velocity factor and matrix (into
 // the second matrix slot when the previous call's matrix still waits).
-__device__ void baro_part2(double* S) {
+__device__ __forceinline__ void baro_part2(double* S) {
 #pragma clang fp contract(off)
   const double* vp = S + MM_S_VEL_PRESS;
   const bool second = S[MM_S_VMAT_ON] != 0.0;
@@ -801,28 +808,45 @@ __device__ void take_nodes_after_vmat(double* S, const double (&nd)[kNodeSums]) 
   if (nd[12] != 0.0) S[MM_S_NAN] = 1.0;
 }
 
-// Sum SoA partials part[q * nb + b] over the blocks b = idx[first + i] (or first + i),
-// i < count, in a fixed order; every thread of the block receives the totals.
-template <int NV>
-__device__ void reduce_blocks(const double* __restrict__ part, int nb,
-                              const int32_t* __restrict__ idx, int first, int count,
-                              double (&out)[NV]) {
+// The force partials fpart[q * nbf + b] over the blocks b = fidx[ff + i] (or ff + i), i < fc,
+// and the node partials npart[q * nbn + nf + i], i < nc, summed in a fixed order: each thread
+// over its strided share in index order, then the wave by the reduce-scatter butterfly, then
+// the waves in order. The loads of both kinds are issued before any sum, and every thread of
+// the block receives the totals.
+__device__ void reduce_force_nodes(const double* __restrict__ fpart, int nbf,
+                                   const int32_t* __restrict__ fidx, int ff, int fc,
+                                   const double* __restrict__ npart, int nbn, int nf, int nc,
+                                   double (&f)[10], double (&nd)[kNodeSums]) {
   constexpr int kWaves = kScalarThreads / 64;
+  constexpr int NV = 10 + kNodeSums;
   __shared__ double red[NV * kWaves];
   __shared__ double tot[NV];
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int i = threadIdx.x; i < count; i += kScalarThreads) {
-    const int b = idx ? idx[first + i] : first + i;
+  for (int i = threadIdx.x; i < fc; i += kScalarThreads) {
+    const int b = fidx ? fidx[ff + i] : ff + i;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] += part[(int64_t)q * nb + b];
+    for (int q = 0; q < 10; ++q) acc[q] += fpart[(int64_t)q * nbf + b];
   }
-  wave_reduce<NV>(acc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
+  for (int i = threadIdx.x; i < nc; i += kScalarThreads) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) red[wave * NV + q] = acc[q];
+    for (int q = 0; q < kNodeSums; ++q) acc[10 + q] += npart[(int64_t)q * nbn + nf + i];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  {
+    double lo[16], hi[NV - 16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) lo[q] = acc[q];
+#pragma unroll
+    for (int q = 16; q < NV; ++q) hi[q - 16] = acc[q];
+    const double tl = wave_reduce_scatter<16>(lo, lane);
+    const double th = wave_reduce_scatter<NV - 16>(hi, lane);
+    const int q = scatter_index(lane);
+    if ((lane & 3) == 0) {
+      red[wave * NV + q] = tl;
+      if (q < NV - 16) red[wave * NV + 16 + q] = th;
+    }
   }
   __syncthreads();
   if (threadIdx.x < NV) {
@@ -832,7 +856,9 @@ __device__ void reduce_blocks(const double* __restrict__ part, int nb,
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < NV; ++q) out[q] = tot[q];
+  for (int q = 0; q < 10; ++q) f[q] = tot[q];
+#pragma unroll
+  for (int q = 0; q < kNodeSums; ++q) nd[q] = tot[10 + q];
   __syncthreads();
 }
 
@@ -1004,25 +1030,29 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
               const int32_t* __restrict__ fsys_ptr, const int32_t* __restrict__ fsys_idx,
               const double* __restrict__ npart, int nbn, const int32_t* __restrict__ nsys_ptr,
               int with_vtens, double* __restrict__ slices, unsigned int* __restrict__ counters) {
+  static_assert(MM_S_COUNT <= kScalarThreads, "one scalar-block entry per thread");
   __shared__ double Sl[MM_S_COUNT];
   __shared__ double tot[kSliceQ];
   __shared__ int is_last;
   const int sys = blockIdx.y;
   const int G = gridDim.x, gi = blockIdx.x;
+  double* Sg = S0 + (int64_t)sys * MM_S_COUNT;
+  // the scalar block is loaded up front, so that its latency hides behind the reduction
+  const double s_own = threadIdx.x < MM_S_COUNT ? Sg[threadIdx.x] : 0.0;
   double f[10], nd[kNodeSums];
-#pragma unroll
-  for (int q = 0; q < 10; ++q) f[q] = 0.0;
-#pragma unroll
-  for (int q = 0; q < kNodeSums; ++q) nd[q] = 0.0;
-  if (phase_uses_force(phase)) {
-    const int b0 = fsys_ptr[sys], cnt = fsys_ptr[sys + 1] - b0;
-    const int lo = (int)((int64_t)cnt * gi / G), hi = (int)((int64_t)cnt * (gi + 1) / G);
-    reduce_blocks<10>(fpart, nbf, fsys_idx, b0 + lo, hi - lo, f);
-  }
-  if (phase_uses_nodes(phase)) {
-    const int b0 = nsys_ptr[sys], cnt = nsys_ptr[sys + 1] - b0;
-    const int lo = (int)((int64_t)cnt * gi / G), hi = (int)((int64_t)cnt * (gi + 1) / G);
-    reduce_blocks<kNodeSums>(npart, nbn, nullptr, b0 + lo, hi - lo, nd);
+  {
+    int ff = 0, fc = 0, nf = 0, nc = 0;
+    if (phase_uses_force(phase)) {
+      const int b0 = fsys_ptr[sys], cnt = fsys_ptr[sys + 1] - b0;
+      ff = b0 + (int)((int64_t)cnt * gi / G);
+      fc = b0 + (int)((int64_t)cnt * (gi + 1) / G) - ff;
+    }
+    if (phase_uses_nodes(phase)) {
+      const int b0 = nsys_ptr[sys], cnt = nsys_ptr[sys + 1] - b0;
+      nf = b0 + (int)((int64_t)cnt * gi / G);
+      nc = b0 + (int)((int64_t)cnt * (gi + 1) / G) - nf;
+    }
+    reduce_force_nodes(fpart, nbf, fsys_idx, ff, fc, npart, nbn, nf, nc, f, nd);
   }
   if (G > 1) {
     if (threadIdx.x == 0) {
@@ -1053,8 +1083,7 @@ scalar_kernel(int phase, double* __restrict__ S0, const double* __restrict__ fpa
 #pragma unroll
     for (int q = 0; q < kNodeSums; ++q) nd[q] = tot[10 + q];
   }
-  double* Sg = S0 + (int64_t)sys * MM_S_COUNT;
-  for (int q = threadIdx.x; q < MM_S_COUNT; q += kScalarThreads) Sl[q] = Sg[q];
+  if (threadIdx.x < MM_S_COUNT) Sl[threadIdx.x] = s_own;
   __syncthreads();
   if (threadIdx.x == 0) {
     // the end of one step and the start of the next, on the same copy of the block
@@ -1177,10 +1206,11 @@ bool reduces(int phase) {
 hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
   const int nbf = (int)(md->field->W / kCellBlock);
   const int G = reduces(phase) ? md->slices : 1;
+  // one system owns every cell block, in order: its block list is the identity
+  const int32_t* cidx = md->nsys == 1 ? nullptr : md->field->sys_cblk_idx;
   hipLaunchKernelGGL(scalar_kernel, dim3(G, md->nsys), dim3(kScalarThreads), 0, s, phase, md->S,
-                     md->field->partials, nbf, md->field->sys_cblk_ptr,
-                     md->field->sys_cblk_idx, md->npart, md->nbn, md->sys_nblk, with_vtens,
-                     md->slice_buf, md->counters);
+                     md->field->partials, nbf, md->field->sys_cblk_ptr, cidx, md->npart, md->nbn,
+                     md->sys_nblk, with_vtens, md->slice_buf, md->counters);
   return hipGetLastError();
 }
 
