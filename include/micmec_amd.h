@@ -296,10 +296,15 @@ void mm_adjoint_destroy(mm_adjoint* a);
  * while the previous call's still waits (steps of one mm_md_step call). */
 #define MM_S_VMAT2 149    /* 9 */
 #define MM_S_VMAT2_ON 158
+/* A pending position transform applied before XMAT: the barostat post call's scaling of a step
+ * that another step of the same call follows, whose positions the next scaling pass brings up
+ * to date (the post call's force evaluation applies it on the fly). */
+#define MM_S_XMAT2_ON 159
 #define MM_S_BCHAIN_MASS 160 /* 8 */
 #define MM_S_BCHAIN_POS 168  /* 8 */
 #define MM_S_BCHAIN_VEL 176  /* 8 */
-#define MM_S_COUNT 192
+#define MM_S_XMAT2 184       /* 9 */
+#define MM_S_COUNT 200
 
 /* ------------------------------------------------------------------------------------------
  * Topology of a grid of cells, built on the device: Grid + build_system (micmec/utils.py:55-263)

@@ -211,6 +211,14 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
   }
 }
 
+// Row vector times 3x3 matrix, without contractions: the position and velocity transforms of
+// the barostat (npt.py:700-702, 725-726) give the same bits in every kernel that applies them.
+__device__ __forceinline__ void vecmat(const double v[3], const double* M, double out[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) out[j] = (v[0] * M[j] + v[1] * M[3 + j]) + v[2] * M[6 + j];
+}
+
 // Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
 // every XCD has its own L2. Neighbouring blocks share cache lines (the cell segments share
 // nodes, the node blocks' record gathers share the lines at their ends), so each XCD is given

@@ -61,14 +61,27 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* smem
 // All 24 loads are issued before any arithmetic, and the shifts are formed with factors 0 or 1
 // (exact products) rather than data-dependent selects, so that no load ends up behind a branch
 // and a full wait; the sums are those of the selects, in the same order.
+// With kXmat every position is first taken through the pending transform xm (vecmat, the
+// product the scaling pass stores).
+template <bool kXmat = false>
 __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
                                                 const double* __restrict__ rv, const int n[8],
-                                                int wrap, double r[8][3]) {
+                                                int wrap, double r[8][3],
+                                                const double* __restrict__ xm = nullptr) {
   double p[8][3];
 #pragma unroll
   for (int k = 0; k < 8; ++k)
 #pragma unroll
     for (int j = 0; j < 3; ++j) p[k][j] = pos[3 * (int64_t)n[k] + j];
+  if (kXmat) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      double t[3];
+      vecmat(p[k], xm, t);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) p[k][j] = t[j];
+    }
+  }
   const double fx = (double)(wrap & 1), fy = (double)((wrap >> 1) & 1),
                fz = (double)((wrap >> 2) & 1);
   double sx[3], sy[3], sz[3];
@@ -104,7 +117,8 @@ __device__ constexpr int kUpperCorner[4] = {3, 5, 6, 7};
 // the single-state instance does not carry the registers of the state combination.
 // kVir: the virial as the sum of the corners' B^T S B (state_energy), which the vertex
 // gradients give as sum_v g_v (x) r_v; with kGrad false the pass then forms no gradient at all
-// (energy and virial only).
+// (energy and virial only), and takes the node positions through the pending transform xmat
+// when that is given (a barostat post call whose position scaling waits).
 template <int kModel, bool kGrad, bool kMulti, bool kVir>
 __global__ void __launch_bounds__(kCellBlock, (kVir && !kGrad && !kMulti) ? 3 : 2)
 cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
@@ -113,7 +127,8 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
             int32_t max_states, int64_t W, const int32_t* __restrict__ cbase, int64_t S,
-            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks) {
+            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks,
+            const double* __restrict__ xmat) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[(kMulti && !kGrad && !kVir) ? kMaxStates * kCellBlock : 1];
   // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
@@ -130,6 +145,9 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   double rv[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) rv[q] = rvb[q];
+  constexpr bool kCanX = kVir && !kGrad;
+  const double* xm =
+      (kCanX && xmat) ? xmat + (blk_sys ? (int64_t)blk_sys[bid] * rvecs_stride : 0) : nullptr;
 
   const int64_t w = (int64_t)bid * kCellBlock + threadIdx.x;
   int n[8];
@@ -152,7 +170,10 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   const int wrap = valid ? cwrap[w] : 0;
   if (valid) {
     double r[8][3];
-    gather_vertices(pos, rv, n, wrap, r);
+    if (kCanX && xm)
+      gather_vertices<true>(pos, rv, n, wrap, r, xm);
+    else
+      gather_vertices(pos, rv, n, wrap, r);
     double ecell;
     if (!kMulti) {
       ecell = tab[10] + state_energy<kModel, kGrad, true, kVir>(tab, r, gacc, 1.0, vir);
@@ -448,11 +469,13 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI, VIR>), dim3(NBL), dim3(kCellBlock), 0, S, \
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
                      a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
-                     a.partials, B0, nb)
+                     a.partials, B0, nb, a.xmat)
   // corner_virial: the virial from the corners; without records the pass then forms no
   // gradient at all
   const bool vir = a.corner_virial != 0;
   const bool kgrad = grad && !(vir && !a.store_records);
+  // only the pass without gradients takes the positions through a pending transform
+  if (a.xmat != nullptr && (kgrad || !vir)) return hipErrorInvalidValue;
   // The instance that takes longer is launched first, on the caller's stream: it then starts
   // right behind the preceding kernel instead of after the side stream's. That is the
   // single-state one, except without gradients and with the corner virial, where the

@@ -140,13 +140,6 @@ __device__ __forceinline__ double node_ekin(const double v[3], double m) {
   return e;
 }
 
-// row vector times 3x3 matrix
-__device__ __forceinline__ void vecmat(const double v[3], const double* M, double out[3]) {
-#pragma clang fp contract(off)
-#pragma unroll
-  for (int j = 0; j < 3; ++j) out[j] = (v[0] * M[j] + v[1] * M[3 + j]) + v[2] * M[6 + j];
-}
-
 // Pending velocity transforms, applied in the order the reference performs them:
 // factor A (thermostat post of the previous half), matrix M (barostat), matrix M2 (the next
 // barostat call, when M still waits), factor B (thermostat pre).
@@ -214,6 +207,12 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
       double p[3], t[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) p[j] = x[3 * n + j];
+      // a pending earlier transform first (the previous post call's, of steps in a row)
+      if (S[MM_S_XMAT2_ON] != 0.0) {
+        vecmat(p, S + MM_S_XMAT2, t);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) p[j] = t[j];
+      }
       vecmat(p, S + MM_S_XMAT, t);
 #pragma unroll
       for (int j = 0; j < 3; ++j) x[3 * n + j] = t[j];
@@ -947,6 +946,10 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       // (it is the kick-drift pass), so a NaN gradient shows here through the virial.
       take_force(S, f, true);
       take_nodes_after_vmat(S, nd);
+      // the scaling pass before this force applied a pending earlier position transform; its
+      // slot is cleared, entries included, as after steps taken one per call
+      S[MM_S_XMAT2_ON] = 0.0;
+      for (int q = 0; q < 9; ++q) S[MM_S_XMAT2 + q] = 0.0;
       update_baro_vel(S, sys, 0, 1);
       baro_call_end(S);
       if (baro_chain_on(S)) baro_chain_step(S);
@@ -997,6 +1000,12 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       take_force(S, f, true);
       take_nodes_after_vmat(S, nd);
       if (!chained) clear_vtransforms(S);
+      // chained, the positions still wait for this call's transform (its force evaluation
+      // applied it on the fly): it moves to the earlier slot, before the next call's
+      if (chained) {
+        for (int q = 0; q < 9; ++q) S[MM_S_XMAT2 + q] = S[MM_S_XMAT + q];
+        S[MM_S_XMAT2_ON] = 1.0;
+      }
       update_baro_vel(S, sys, 1, 1);
       if (baro_langevin(S)) {
         baro_call_end(S);
@@ -1219,9 +1228,11 @@ hipError_t scalar(mm_md* md, int phase, int with_vtens, hipStream_t s) {
 // (gather_kick_kernel): kGatherNone leaves out the records altogether, so the cell pass skips
 // them too, and without scale_v there is then no node pass at all (energy and virial only).
 hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = false,
-                 int gmode = kGatherStore, bool corner_virial = false) {
+                 int gmode = kGatherStore, bool corner_virial = false,
+                 const double* xmat = nullptr) {
   ForceArgs a = force_args(md);
   a.corner_virial = corner_virial;
+  a.xmat = xmat;
   if (gmode == kGatherNone) {
     if (kick) return hipErrorInvalidValue;
     a.store_records = 0;
@@ -1276,16 +1287,19 @@ hipError_t gather_drift(mm_md* md, bool lt, hipStream_t s) {
 // The barostat call MTKBarostat.baro after its first scalar phase: positions scaled, force,
 // then the second barostat-velocity update. The pre call (pre_call) leaves its records for
 // gather_drift, which applies its velocity transform with the kick; the post call's transform
-// is applied by its gather (scale_v) or stays pending.
+// is applied by its gather (scale_v) or stays pending. defer_x (a post call without records):
+// the positions are not scaled in a pass of their own; the cell pass takes them through the
+// transform as it reads them, and the next scaling pass applies it before its own.
 hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, bool scale_v,
                      int gmode = kGatherStore, bool corner_virial = false,
-                     bool pre_call = false) {
-  MM_TRY(node(md, kScalePos, s));
+                     bool pre_call = false, bool defer_x = false) {
+  if (!defer_x) MM_TRY(node(md, kScalePos, s));
   if (pre_call) {
     ForceArgs a = force_args(md);
     MM_TRY(launch_cells(a, s));
   } else {
-    MM_TRY(force(md, false, false, s, scale_v, gmode, corner_virial));
+    MM_TRY(force(md, false, false, s, scale_v, gmode, corner_virial,
+                 defer_x ? md->S + MM_S_XMAT : nullptr));
   }
   MM_TRY(scalar(md, end_phase, 1, s));
   return hipSuccess;
@@ -1314,11 +1328,13 @@ hipError_t enqueue_step(mm_md* md, hipStream_t s, bool lean = false, bool pre = 
     MM_TRY(force(md, true, lt, s, false, lean ? kGatherKeep : kGatherStore));
     MM_TRY(scalar(md, kPhMainEndNpt, 1, s));
     // the post call's velocity transform stays pending in a lean step (its node sums follow
-    // from the ones before it), so its force evaluation needs no node pass
-    // and its virial comes from the corners (B^T S B, no vertex gradient needed), in the
-    // last step too, so that lean and full steps agree to the bit
+    // from the ones before it), and so does its position scaling (the cell pass applies it
+    // as it reads the positions, the next step's scaling pass stores it), so its force
+    // evaluation has no node pass at all; its virial comes from the corners (B^T S B, no
+    // vertex gradient needed), in the last step too, so that lean and full steps agree to
+    // the bit
     MM_TRY(baro_call(md, lean ? kPhBaroEndPostPre : kPhBaroEndPost, s, !lean,
-                     lean ? kGatherNone : kGatherStore, true));
+                     lean ? kGatherNone : kGatherStore, true, false, lean));
   } else if (md->thermo == MM_THERMO_NHC) {
     if (pre) MM_TRY(scalar(md, kPhNvtPre, 1, s));
     MM_TRY(node(md, kKickDrift, s));
