@@ -38,6 +38,7 @@ free energies and kT) are mapped here to (cv, h0, free, teff).
 """
 
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -206,12 +207,14 @@ class DiffField(object):
         unmodified tensors again (the backward pass after its forward) is skipped; a change is
         recognised by the tensors' version counters, which in-place operations on the tensors
         (or their detached views) advance but writes through ``.data`` do not: after such a
-        write, set ``field._uploaded = None`` to force the upload."""
+        write, set ``field._uploaded = None`` to force the upload. The record holds weak
+        references only, so that it keeps no autograd history alive."""
         ts = params.tensors()
-        key = (tuple(ts), tuple(x._version for x in ts), params.nstates.tobytes())
+        key = (tuple(weakref.ref(x) for x in ts), tuple(x._version for x in ts),
+               params.nstates.tobytes())
         last = getattr(self, "_uploaded", None)
         if last is not None and len(last[0]) == len(ts) and \
-                all(a is b for a, b in zip(last[0], key[0])) and last[1:] == key[1:]:
+                all(r() is x for r, x in zip(last[0], ts)) and last[1:] == key[1:]:
             return
         kw = dict(dtype=torch.float64, device=self.device)
         cv, h0, free, teff = [x.detach().to(**kw).contiguous() for x in ts]
@@ -458,37 +461,69 @@ def fit_workload(count=64, n=16, seed=3, c_noise=0.10, device=None, model="corne
     return problem, init, target
 
 
-def fit(problem, init, steps, lr_cv=1e-2, lr_h0=1e-4, callback=None):
+def fit(problem, init, steps, lr_cv=1e-2, lr_h0=1e-4, callback=None, graph=True):
     """Adam on relative corrections of the elasticity matrices and rest cells:
     cv = cv_init * (1 + a) with a symmetric, h0 = h0_init * (1 + b). Returns the loss history
-    and the final CellParams."""
+    and the final CellParams.
+
+    graph: after three eager steps, one whole step (parameter table, forward force pass,
+    loss, backward adjoint pass, Adam update) is captured in a CUDA graph and replayed for the
+    remaining steps, so that the host launches one graph per step instead of some sixty
+    kernels; the steps are the same kernels on the same buffers either way."""
     cv0, h00 = init.cv.detach(), init.h0.detach()
     free, teff = init.free.detach(), init.teff.detach()
     a = torch.zeros_like(cv0, requires_grad=True)
     b = torch.zeros_like(h00, requires_grad=True)
     groups = [{"params": [a], "lr": lr_cv}, {"params": [b], "lr": lr_h0}]
+    graph = bool(graph) and a.is_cuda and steps > 3
     try:
-        # one fused update kernel per group instead of a dozen foreach launches
-        opt = torch.optim.Adam(groups, fused=True)
+        # one fused update kernel per group instead of a dozen foreach launches; capturable
+        # keeps the step count on the device, as a graph replay needs
+        opt = torch.optim.Adam(groups, fused=True, capturable=graph)
     except (RuntimeError, TypeError, ValueError):
-        opt = torch.optim.Adam(groups)
+        opt = torch.optim.Adam(groups, capturable=graph)
 
     def params_of(a, b):
         asym = 0.5 * (a + a.transpose(-1, -2))
         return CellParams(cv0 * (1.0 + asym), h00 * (1.0 + b), free, teff, init.nstates,
                           init.type_ids)
 
-    losses = []
-    for it in range(steps):
-        opt.zero_grad()
+    def one_step():
         loss = problem.loss(params_of(a, b))
         loss.backward()
         opt.step()
+        return loss
+
+    losses = []
+
+    def record(it, loss):
         # the history stays on the device (no synchronisation per step) unless a callback
         # wants the value now
-        losses.append(loss.detach())
+        losses.append(loss.detach().clone())
         if callback is not None:
             callback(it, float(losses[-1]))
+
+    if not graph:
+        for it in range(steps):
+            opt.zero_grad()
+            record(it, one_step())
+    else:
+        # eager steps on a side stream first (the library's and torch's lazy set-up, the
+        # optimizer state), as capture requires
+        side = torch.cuda.Stream(device=a.device)
+        side.wait_stream(torch.cuda.current_stream(a.device))
+        with torch.cuda.stream(side):
+            for it in range(3):
+                opt.zero_grad(set_to_none=True)
+                record(it, one_step())
+        torch.cuda.current_stream(a.device).wait_stream(side)
+        opt.zero_grad(set_to_none=True)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            static_loss = one_step()
+        for it in range(3, steps):
+            g.replay()
+            record(it, static_loss)
     with torch.no_grad():
         final = params_of(a, b)
     losses = torch.stack(losses).cpu().tolist() if losses else []
