@@ -31,17 +31,17 @@
 
 namespace mm {
 
-// Deterministic block reduction (fixed shuffle tree per wave, waves in order); the block's
-// totals are written SoA to out[q * stride].
+// Deterministic block reduction (the reduce-scatter butterfly per wave, 17 exchanges for up to
+// 16 values, then the waves in order); the block's totals are written SoA to out[q * stride].
 template <int NV, int BS>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* smem, double* out,
                                                    int64_t stride) {
-  wave_reduce<NV>(v);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) smem[wave * NV + q] = v[q];
+  {
+    const double t = wave_reduce_scatter<NV>(v, lane);
+    const int q = scatter_index(lane);
+    if ((lane & 3) == 0 && q < NV) smem[wave * NV + q] = t;
   }
   __syncthreads();
   if (threadIdx.x < NV) {
