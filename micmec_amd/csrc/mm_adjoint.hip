@@ -82,24 +82,15 @@ template <> struct IsDual<Dual> { static constexpr bool value = true; };
 // Lane exchanges inside groups of eight lanes, as DPP moves (VALU, no LDS round trip): lane l
 // receives the value of lane l ^ mask. Masks 1 and 2 are quad permutations ([1,0,3,2] and
 // [2,3,0,1]); mask 4 swaps the two 4-lane banks of each half row, as a row shift left by 4
-// written to banks 0 and 2 and a row shift right by 4 written to banks 1 and 3. A double
-// moves as its two 32-bit halves.
-template <int kCtrl, int kBanks>
-__device__ __forceinline__ double dpp64(double old, double v) {
-  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, kCtrl, 0xF, kBanks, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), kCtrl, 0xF, kBanks,
-                                             false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
-                                          (unsigned long long)(unsigned)lo));
-}
+// written to banks 0 and 2 and a row shift right by 4 written to banks 1 and 3 (dpp_f64: a
+// double moves as its two 32-bit halves).
 template <int kMask>
 __device__ __forceinline__ double xs8(double v) {
   static_assert(kMask == 1 || kMask == 2 || kMask == 4, "exchanges inside 8 lanes");
-  if (kMask == 1) return dpp64<0xB1, 0xF>(v, v);
-  if (kMask == 2) return dpp64<0x4E, 0xF>(v, v);
-  const double t = dpp64<0x104, 0x5>(v, v);
-  return dpp64<0x114, 0xA>(t, v);
+  if (kMask == 1) return dpp_f64<0xB1, 0xF>(v, v);
+  if (kMask == 2) return dpp_f64<0x4E, 0xF>(v, v);
+  const double t = dpp_f64<0x104, 0x5>(v, v);
+  return dpp_f64<0x114, 0xA>(t, v);
 }
 __device__ __forceinline__ double xs(double v, int mask) {
   return mask == 1 ? xs8<1>(v) : mask == 2 ? xs8<2>(v) : xs8<4>(v);

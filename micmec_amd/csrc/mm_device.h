@@ -230,36 +230,87 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return x * q + (x < r ? x : r) + i;
 }
 
-// One step of wave_reduce_scatter: the lane keeps the upper H of its 2H values when bit BIT of its
-// lane index is set, the lower H otherwise, and adds its partner's copy of them.
-template <int H, int BIT>
-__device__ __forceinline__ void rs_step(double (&a)[16], int lane) {
-  const bool up = (lane & BIT) != 0;
-#pragma unroll
-  for (int i = 0; i < H; ++i) {
-    const double send = up ? a[i] : a[H + i];
-    const double keep = up ? a[H + i] : a[i];
-    a[i] = keep + __shfl_xor(send, BIT, 64);
-  }
+// 64-bit DPP move (two 32-bit moves): the lanes that kBanks enables take v from the lane kCtrl
+// names (row_shl:N reads lane + N, row_shr:N lane - N, quad_perm within four lanes), the others
+// keep old. A register-to-register exchange, no LDS round trip.
+template <int kCtrl, int kBanks>
+__device__ __forceinline__ double dpp_f64(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, kCtrl, 0xF, kBanks, false);
+  const int hi =
+      __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), kCtrl, 0xF, kBanks, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
 }
 
-// Wave sums of up to 16 values by a reduce-scatter butterfly (17 shuffles instead of 6 NV):
-// returns in lane l the wave total of value q(l) = 8 b5 + 4 b4 + 2 b3 + b2 (bk = bit k of l);
-// values q >= NV are zero. Fixed pattern, so the sums are deterministic.
+// Row swap of two registers (v_permlane32_swap / v_permlane16_swap): with kDist 32 lanes 32-63
+// of x trade places with lanes 0-31 of y, with kDist 16 the odd 16-lane rows of x with the even
+// rows of y. Afterwards a lane whose bit kDist is clear holds its own x in x and its partner's
+// (lane ^ kDist) x in y; a lane whose bit is set holds its own y in y and its partner's y in x.
+template <int kDist>
+__device__ __forceinline__ void row_swap(double& x, double& y) {
+  static_assert(kDist == 16 || kDist == 32, "row swaps across 16 or 32 lanes");
+  const unsigned long long xb = (unsigned long long)__double_as_longlong(x);
+  const unsigned long long yb = (unsigned long long)__double_as_longlong(y);
+  const unsigned xl = (unsigned)xb, xh = (unsigned)(xb >> 32);
+  const unsigned yl = (unsigned)yb, yh = (unsigned)(yb >> 32);
+  unsigned xl2, xh2, yl2, yh2;
+  if (kDist == 32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    xl2 = lo[0], yl2 = lo[1], xh2 = hi[0], yh2 = hi[1];
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    xl2 = lo[0], yl2 = lo[1], xh2 = hi[0], yh2 = hi[1];
+  }
+  x = __longlong_as_double((long long)(((unsigned long long)xh2 << 32) | xl2));
+  y = __longlong_as_double((long long)(((unsigned long long)yh2 << 32) | yl2));
+}
+
+// Wave sums of up to 16 values by a reduce-scatter butterfly: returns in lane l the wave total
+// of value q(l) = 8 b5 + 4 b4 + 2 b3 + b2 (bk = bit k of l); values q >= NV are zero. At each
+// level a lane keeps the upper half of its values when its bit is set, the lower half
+// otherwise, and adds its partner's copy of them: across 32 and 16 lanes by row swaps, across
+// 8 and 4 lanes by two bank-masked DPP row shifts, the last two levels (a plain butterfly on
+// one value) by quad permutations. Every exchange stays in registers; the pairs and the order
+// of every addition are fixed, so the sums are deterministic.
 template <int NV>
 __device__ __forceinline__ double wave_reduce_scatter(const double (&v)[NV], int lane) {
-  static_assert(NV <= 16, "at most 16 values");
+  static_assert(NV >= 1 && NV <= 16, "1 to 16 values");
+  constexpr int L1 = NV < 8 ? NV : 8, L2 = L1 < 4 ? L1 : 4, L3 = L2 < 2 ? L2 : 2;
   double a[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) a[q] = q < NV ? v[q] : 0.0;
-  rs_step<8, 32>(a, lane);
-  rs_step<4, 16>(a, lane);
-  rs_step<2, 8>(a, lane);
-  rs_step<1, 4>(a, lane);
-  double t = a[0];
-  t += __shfl_xor(t, 2, 64);
-  t += __shfl_xor(t, 1, 64);
-  return t;
+  // lane ^ 32, pairs (i, 8 + i); a pair of zeros stays zero
+#pragma unroll
+  for (int i = 0; i < L1; ++i) {
+    double x = a[i], y = a[8 + i];
+    row_swap<32>(x, y);
+    a[i] = x + y;
+  }
+  // lane ^ 16, pairs (i, 4 + i)
+#pragma unroll
+  for (int i = 0; i < L2; ++i) {
+    double x = a[i], y = a[4 + i];
+    row_swap<16>(x, y);
+    a[i] = x + y;
+  }
+  // lane ^ 8, pairs (i, 2 + i): lanes 0-7 of a row read lane + 8's a[i], lanes 8-15 lane - 8's
+  // a[2 + i]
+  const bool up8 = (lane & 8) != 0;
+#pragma unroll
+  for (int i = 0; i < L3; ++i) {
+    const double t = dpp_f64<0x118, 0xC>(dpp_f64<0x108, 0x3>(0.0, a[i]), a[2 + i]);
+    a[i] = (up8 ? a[2 + i] : a[i]) + t;
+  }
+  // lane ^ 4, the pair (0, 1): banks 0 and 2 read lane + 4, banks 1 and 3 lane - 4
+  const bool up4 = (lane & 4) != 0;
+  const double t = dpp_f64<0x114, 0xA>(dpp_f64<0x104, 0x5>(0.0, a[0]), a[1]);
+  double s = (up4 ? a[1] : a[0]) + t;
+  s += dpp_f64<0x4E, 0xF>(s, s);  // lane ^ 2
+  s += dpp_f64<0xB1, 0xF>(s, s);  // lane ^ 1
+  return s;
 }
 
 __device__ __forceinline__ int scatter_index(int lane) {
