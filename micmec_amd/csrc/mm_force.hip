@@ -270,13 +270,15 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
     // The upper face (corners 3, 5, 6, 7) of the slot below is added to this slot's lower face
     // (corners 0, 1, 2, 4) when that slot holds the -z neighbour (wrap bit 3); the lower faces
     // go to their class planes, the upper faces that were not folded (wrap bit 4 clear) to the
-    // wave's compact slots, in lane order.
+    // wave's compact slots, in lane order. The move from lane - 1 is a DPP wave shift (lane 0,
+    // which never folds, keeps its own value).
     const bool fold = (wrap & 8) != 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const double t = __shfl_up(gacc[kUpperCorner[q]][j], 1, 64);
+        const double up = gacc[kUpperCorner[q]][j];
+        const double t = dpp_f64<0x138, 0xF>(up, up);  // wave_shr:1
         const double lo = gacc[kLowerCorner[q]][j];
         gacc[kLowerCorner[q]][j] = fold ? lo + t : lo;
       }
