@@ -375,12 +375,19 @@ __device__ void matmul3(const double* A, const double* B, double* C) {
   for (int i = 0; i < 9; ++i) C[i] = T[i];
 }
 
+// 1 / k for the Horner steps of expm_sym_half (correctly rounded, as the division gives).
+__device__ constexpr double kInvK[15] = {0.0,       1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,
+                                         1.0 / 5,   1.0 / 6,    1.0 / 7,  1.0 / 8,  1.0 / 9,
+                                         1.0 / 10,  1.0 / 11,   1.0 / 12, 1.0 / 13, 1.0 / 14};
+
 // R = exp(sign * A * dt / 2) for symmetric A: the matrix the reference forms from
 // eigh(A) = (w, Q) as Q diag(exp(sign w dt / 2)) Q^T (npt.py:686-690, 709-726), here by scaling
-// and squaring. X = sign * A * dt / 2 is halved s times until ||X||_inf <= 1/2, exp(X) is the
-// Taylor polynomial to the X^14 term in Horner form (remainder below 0.5^15 / 15! < 3e-17
-// relative), and the result is squared s times. Multiply-adds only: no square roots or
-// divisions on the serial path.
+// and squaring. X = sign * A * dt / 2 is halved s times until r = ||X||_inf <= 1/2, exp(X) is
+// the Taylor polynomial in Horner form to the smallest degree m <= 14 whose first omitted term
+// bound r^(m+1) / (m+1)! is below 1e-17 (the remainder is at most that times e^r against
+// ||exp(X)|| >= e^-r: below 3e-17 relative, the bound of the full degree 14 at r = 1/2; a
+// barostat velocity times dt / 2 is of order 1e-5, where m = 3), and the result is squared s
+// times. Multiply-adds only: no square roots or divisions on the serial path.
 __device__ void expm_sym_half(const double* A, double sign, double dt, double* R) {
   const double h = (sign * dt) / 2.0;
   double X[9];
@@ -403,13 +410,19 @@ __device__ void expm_sym_half(const double* A, double sign, double dt, double* R
   }
 #pragma unroll
   for (int q = 0; q < 9; ++q) X[q] *= scale;
+  const double r = nrm * scale;
+  int m = 1;
+  double bound = (r * r) / 2.0;
+  while (m < 14 && bound > 1e-17) {
+    ++m;
+    bound *= r / (double)(m + 1);
+  }
   double P[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) P[q] = (q % 4 == 0) ? 1.0 : 0.0;
-#pragma unroll
-  for (int k = 14; k >= 1; --k) {
+  for (int k = m; k >= 1; --k) {
     // P <- I + X P / k
-    const double inv = 1.0 / (double)k;
+    const double inv = kInvK[k];
     double T[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -465,9 +478,11 @@ __device__ double nhc_chain(double* S, double* ekin_io, bool has_g1, double g1,
     if (k == L - 1) {
       vel[k] += (g * dt) / 4.0;
     } else {
-      vel[k] *= exp(((-vel[k + 1]) * dt) / 8.0);
+      // the same factor before and after the force term (vel[k + 1] does not change between)
+      const double e = exp(((-vel[k + 1]) * dt) / 8.0);
+      vel[k] *= e;
       vel[k] += (g * dt) / 4.0;
-      vel[k] *= exp(((-vel[k + 1]) * dt) / 8.0);
+      vel[k] *= e;
     }
   };
   double ekin = *ekin_io;
@@ -570,11 +585,12 @@ __device__ __forceinline__ void update_baro_vel(double* S, int sys, int post, in
   const bool has = S[MM_S_HAS_CHAINVEL0] != 0.0;
   const double cv0 = S[MM_S_CHAINVEL0];
   const double fric = lb ? exp((-dt) / (8.0 * S[MM_S_LB_TIMECON])) : 1.0;
+  // the chain's damping factor, applied before and after the force term
+  const double fchain = has ? exp(((-dt) * cv0) / 8.0) : 1.0;
   if (lb)
     for (int i = 0; i < nv; ++i) vp[i] *= fric;
   if (has) {
-    const double f = exp(((-dt) * cv0) / 8.0);
-    for (int i = 0; i < nv; ++i) vp[i] *= f;
+    for (int i = 0; i < nv; ++i) vp[i] *= fchain;
   }
   double P[9], G[9];
   for (int i = 0; i < 9; ++i) P[i] = S[MM_S_SVV + i] - S[MM_S_VTENS + i];
@@ -620,8 +636,7 @@ __device__ __forceinline__ void update_baro_vel(double* S, int sys, int post, in
     for (int i = 0; i < nv; ++i) vp[i] *= fric;
   }
   if (has) {
-    const double f = exp(((-dt) * cv0) / 8.0);
-    for (int i = 0; i < nv; ++i) vp[i] *= f;
+    for (int i = 0; i < nv; ++i) vp[i] *= fchain;
   }
 }
 
