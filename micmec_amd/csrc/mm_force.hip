@@ -479,10 +479,11 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   // only the pass without gradients takes the positions through a pending transform
   if (a.xmat != nullptr && (kgrad || !vir)) return hipErrorInvalidValue;
   // The instance that takes longer is launched first, on the caller's stream: it then starts
-  // right behind the preceding kernel instead of after the side stream's. That is the
-  // single-state one, except without gradients and with the corner virial, where the
-  // single-state instance runs three waves a SIMD and the multistate one two.
-  const bool multi_lead = vir && !kgrad;
+  // right behind the preceding kernel instead of after the side stream's. That is the one
+  // with more state evaluations (a multistate block counted as two), and the multistate one
+  // without gradients and with the corner virial, where the single-state instance runs three
+  // waves a SIMD and the multistate one two.
+  const bool multi_lead = (vir && !kgrad) || 2 * (int64_t)mn > (int64_t)sn;
 #define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
   do {                                                                                       \
     if (multi_lead) {                                                                        \
