@@ -7,12 +7,14 @@
 //   pre  = MTK.baro (npt.py:653-736) then NHC chain (nvt.py:410-451);
 //   post = NHC chain then MTK.baro. Each MTK.baro call evaluates the force once more.
 // Node-wise work runs in wide kernels; every scalar update (chain, barostat velocity, 3x3
-// eigendecompositions, energies, conserved-quantity tracking) runs in a single-block kernel on
-// the scalar block S, between the node kernels, so a whole step can be captured in a HIP graph.
-// Velocity/position transforms produced by a scalar phase (thermostat factor, barostat
-// rotation) are stored in S and applied by the next node pass that touches the array.
-// The Langevin thermostat's noise is drawn inside the node passes that already touch the
-// velocities (pre: the kick-drift pass, post: the gather-kick pass), from a counter-based
+// symmetric exponentials, energies, conserved-quantity tracking) runs in a small kernel on the
+// scalar block S between the node kernels (its blocks reduce the partial sums, the last one
+// runs the phase), so a whole step can be captured in a HIP graph. Velocity/position
+// transforms produced by a scalar phase (thermostat factors, barostat matrices) are stored in
+// S and applied by the next pass that reads the array; kinetic sums after a pending barostat
+// matrix follow from the sums before it. The Langevin thermostat's noise is drawn inside the
+// node passes that already touch the velocities (pre: the kick-drift pass, with a barostat
+// the gather that is the kick-drift; post: the gather-kick pass), from a counter-based
 // generator keyed by node and hook call, so it costs no extra pass over the nodes.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -57,9 +59,9 @@ enum Phase {
   kPhNvtPre = 3,         // NHC pre
   kPhNvtEnd = 4,         // force + node sums; NHC post; properties
   kPhBaroPre = 5,        // TBC pre, barostat: v_g update, position/domain and velocity transforms
-  kPhBaroEndPre = 7,     // force + node sums after the velocity transform; v_g update; NHC pre
+  kPhBaroEndPre = 7,     // force; kinetic sums after the pending velocity matrix; v_g; NHC pre
   kPhMainEndNpt = 8,     // force + node sums; NHC post; barostat transforms of the post call
-  kPhBaroEndPost = 9,    // force + node sums; v_g update; econs; properties
+  kPhBaroEndPost = 9,    // force (+ gather NaN flag); kinetic sums after the matrix; v_g; econs
   kPhRmsd = 10,          // rmsd_gpos / rmsd_delta
   kPhApplied = 11,       // pending transforms were applied
   kPhBaroEndPostPre = 12,  // kPhBaroEndPost, then the next step's kPhBaroPre (steps in a row)
@@ -247,8 +249,9 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // v += a dt / 2 (verlet.py:152-153) and the node sums of the new velocities; with kLT also
 // the Langevin thermostat's post update (the node sums then hold the ekin before it in [13]).
 // With kScaleV instead (a barostat call, npt.py:704-735), the pending velocity transforms are
-// applied (the barostat's exp(-(v_g + tr(v_g)/ndof) dt / 2), known before its force) and the
-// node sums of the transformed velocities taken, in the same pass.
+// applied (the barostat's exp(-(v_g + tr(v_g)/ndof) dt / 2), known before its force) in the
+// same pass; the phase after it takes the kinetic sums from those before the matrix, so this
+// pass leaves only the NaN flag of the gradient.
 // gmode: kGatherStore sums the records and stores gpos; kGatherKeep sums them for the kick only
 // (the step overwrites gpos before anything reads it); kGatherNone (kScaleV only) reads no
 // records: the force evaluation fed only the energy and the virial.
@@ -319,7 +322,6 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       apply_vtransform(S, vv);
 #pragma unroll
       for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
-      node_vv(vv, m[n], acc);
     }
     if (kDrift) {
 #pragma clang fp contract(off)
@@ -348,7 +350,7 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       double lt[1] = {acc[0]};
       block_sum_store_wide<1>(lt, partials + (int64_t)14 * gridDim.x, blk);
     }
-  } else if (kKick || kScaleV) {
+  } else if (kKick) {
     block_sum_store_wide<NA>(acc, partials, blk);
   } else {
     // only the NaN flag: any lane's flag, per block

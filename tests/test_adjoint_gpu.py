@@ -367,6 +367,20 @@ def test_fit_graph_replays_the_eager_steps_bitwise():
     assert le == lg
     for a, b in zip(fe.tensors(), fg.tensors()):
         assert torch.equal(a, b)
+    # replays with the host reading each loss (a synchronisation) and eager work between them:
+    # every replay against the eager step (the graph holds torch's reductions, which clear
+    # their own scratch memory)
+    seen = []
+
+    def between(it, value):
+        seen.append(value)
+        torch.cuda.synchronize()
+        _ = (pos * 2.0).sum().item()
+
+    ls, fs = fit(problem, field.initial_params(), 12, graph=True, callback=between)
+    assert ls == le and seen == le
+    for a, b in zip(fe.tensors(), fs.tensors()):
+        assert torch.equal(a, b)
 
 
 def test_device_parameter_table_equals_host_table(setup):
