@@ -203,6 +203,19 @@ int mm_adjoint_compute(mm_adjoint* a, const double* pos, const double* dpos, con
 int mm_adjoint_vjp(mm_adjoint* a, const double* pos, const double* rvecs,
                    const int32_t* node_sys, const double* gE, const double* gG, const double* gV,
                    double* sys_out, double* g_pos, double* params, void* stream);
+
+/* Gradients of the parameter tensors from parameter entries as mm_adjoint_compute lays them
+ * out: entry q of row r at entries[(r * npar + q) * stride], plus the next double when
+ * add_next (stride 2, add_next 1: the value and tangent rows of params summed, the cotangent of
+ * a vector-Jacobian product). With h0 [nrows][max_states][9] (row-major, the rest cells the
+ * table was built from): g_cv [nrows][max_states][36] (the symmetric 6x6 Voigt gradient),
+ * g_h0 [nrows][max_states][9] = -P^T (dE/dP) P^T + (dE/dw) w P^T with P = h0^-1 (cofactors)
+ * and w = det(h0)/8 (corner8) or det(h0), g_free [nrows][max_states], all zero for the states
+ * a row does not have, and g_teff [nrows] = boltzmann dE/d(kT). Device pointers, one kernel.
+ * No reference counterpart (differentiation form). */
+int mm_adjoint_param_grads(mm_adjoint* a, const double* entries, int64_t stride, int add_next,
+                           const double* h0, double boltzmann, double* g_cv, double* g_h0,
+                           double* g_free, double* g_teff, void* stream);
 void mm_adjoint_destroy(mm_adjoint* a);
 
 /* ------------------------------------------------------------------------------------------

@@ -60,6 +60,8 @@ _native.register_signature("mm_field_set_params", [_P, _P, _P, _P, _P, _P, _f64,
 _native.register_signature("mm_adjoint_create", [ctypes.POINTER(_P), _P, ctypes.POINTER(_i64)])
 _native.register_signature("mm_adjoint_compute", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 _native.register_signature("mm_adjoint_vjp", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+_native.register_signature("mm_adjoint_param_grads",
+                           [_P, _P, _i64, ctypes.c_int, _P, _f64, _P, _P, _P, _P, _P])
 
 NPS = 32       # parameter entries per (type, state) of the adjoint kernel
 SYSQ = 19      # energy, virial (9), domain-vector gradient (9)
@@ -95,17 +97,6 @@ def tensor_from_voigt(cv):
                 for l in range(3):
                     out[..., i, j, k, l] = cv[..., inv[(i, j)], inv[(k, l)]]
     return out
-
-
-def _inv3_det(h):
-    """Inverses and determinants of [..., 3, 3] matrices by cofactors: with rows r0, r1, r2 the
-    cofactor rows are r1 x r2, r2 x r0, r0 x r1 and det = r0 . (r1 x r2) (a few batched tensor
-    operations: no library solve and no synchronisation)."""
-    r0, r1, r2 = h[..., 0, :], h[..., 1, :], h[..., 2, :]
-    cof = torch.stack([torch.linalg.cross(r1, r2), torch.linalg.cross(r2, r0),
-                       torch.linalg.cross(r0, r1)], -2)
-    det = (r0 * cof[..., 0, :]).sum(-1)
-    return cof.transpose(-1, -2) / det[..., None, None], det
 
 
 class CellParams(object):
@@ -305,48 +296,36 @@ class DiffField(object):
         _native.check(rc, "mm_adjoint_vjp")
         return dict(g_pos=g_pos, grvecs=sys_out[:, 10:19].reshape(-1, 3, 3),
                     dgrvecs=sys_out[:, SYSQ + 10:].reshape(-1, 3, 3), params=params[:, :, 0],
-                    dparams=params[:, :, 1])
+                    dparams=params[:, :, 1], raw_params=params)
 
-    def param_grads(self, entries, params):
-        """Map raw kernel entries [nrows, npar] to gradients of (cv, h0, free, teff)."""
-        cv, h0, free, teff = [x.detach() for x in params.tensors()]
+    def param_grads(self, entries, params, add_next=False):
+        """Map raw kernel entries [nrows, npar] to gradients of (cv, h0, free, teff), in one
+        kernel (``mm_adjoint_param_grads``: the symmetric Voigt gradient, dE/dh0 from dE/dP and
+        dE/dw through the cofactor inverse of h0, the states a row lacks zeroed). With
+        ``add_next`` the entries are the [nrows, npar, 2] value and tangent rows of an adjoint
+        launch, summed on the way."""
         T, S = self.nrows, self.max_states
-        ent = entries[:, : S * NPS].reshape(T, S, NPS)
-        # the symmetric 6x6 from its 21 upper-triangle entries, in one gather
-        g_cv = ent[:, :, self._voigt_columns()].reshape(T, S, 6, 6)
-        gP = ent[:, :, 21:30].reshape(T, S, 3, 3)
-        gw = ent[:, :, 30]
-        inv, det = _inv3_det(h0)
-        Pt = inv.transpose(-1, -2).contiguous()
-        w = det / 8.0 if self.model == "corner8" else det
-        g_h0 = -(Pt @ gP @ Pt) + (gw * w)[..., None, None] * Pt
-        g_free = ent[:, :, 31].clone()
-        g_teff = entries[:, S * NPS] * units.boltzmann
-        mask = self._state_mask(params.nstates)
-        g_cv = g_cv * mask[..., None, None]
-        g_h0 = g_h0 * mask[..., None, None]
-        g_free = g_free * mask
+        kw = dict(dtype=torch.float64, device=self.device)
+        # the state counts the kernel masks with are the field's: those of params
+        self.set_params(params)
+        ent = entries.contiguous()
+        if ent.dtype != torch.float64 or ent.device != self.device or \
+                ent.numel() != T * self.npar * (2 if add_next else 1):
+            raise ValueError("entries must be float64 [%d, %d%s] on %s"
+                             % (T, self.npar, ", 2" if add_next else "", self.device))
+        h0 = params.h0.detach().to(**kw).contiguous()
+        if h0.shape != (T, S, 3, 3):
+            raise ValueError("h0 must be [%d, %d, 3, 3]" % (T, S))
+        g_cv = torch.empty((T, S, 6, 6), **kw)
+        g_h0 = torch.empty((T, S, 3, 3), **kw)
+        g_free = torch.empty((T, S), **kw)
+        g_teff = torch.empty((T,), **kw)
+        rc = self.lib.mm_adjoint_param_grads(
+            self._adj, _native.ptr(ent), 2 if add_next else 1, 1 if add_next else 0,
+            _native.ptr(h0), float(units.boltzmann), _native.ptr(g_cv), _native.ptr(g_h0),
+            _native.ptr(g_free), _native.ptr(g_teff), _native.stream_ptr())
+        _native.check(rc, "mm_adjoint_param_grads")
         return g_cv, g_h0, g_free, g_teff
-
-    def _voigt_columns(self):
-        """[36] device index: entry (I, J) of the 6x6 Voigt matrix -> its column among the 21
-        upper-triangle parameter entries (cached)."""
-        cols = self.__dict__.get("_vcols")
-        if cols is None:
-            idx = np.zeros((6, 6), dtype=np.int64)
-            for q, (I, J) in enumerate(voigt_pairs()):
-                idx[I, J] = idx[J, I] = q
-            cols = self._vcols = torch.tensor(idx.ravel(), device=self.device)
-        return cols
-
-    def _state_mask(self, nstates):
-        """[nrows, max_states] device mask of the states each row has (cached)."""
-        key = np.asarray(nstates, dtype=np.int32).tobytes()
-        cache = self.__dict__.setdefault("_masks", {})
-        if key not in cache:
-            cache[key] = torch.tensor(np.arange(self.max_states)[None, :] <
-                                      np.asarray(nstates)[:, None], device=self.device)
-        return cache[key]
 
     def __del__(self):
         lib = getattr(self, "lib", None)
@@ -396,7 +375,7 @@ class _MicMecEnergy(torch.autograd.Function):
         if gV is not None:
             gV = gV.contiguous().reshape(-1, 3, 3)
             g_rv = g_rv + (grv.unsqueeze(3) * gV.unsqueeze(1)).sum(2)
-        g_cv, g_h0, g_free, g_teff = field.param_grads(out["params"] + out["dparams"], params)
+        g_cv, g_h0, g_free, g_teff = field.param_grads(out["raw_params"], params, add_next=True)
         return (None, None, g_pos, g_rv.reshape(ctx.rv_shape), g_cv, g_h0, g_free, g_teff)
 
 

@@ -564,6 +564,67 @@ vjp_position_kernel(const double* __restrict__ gpos, const double* __restrict__ 
 
 using namespace mm;
 
+// Gradients of (Cv, h0, free energy, T_eff) from the parameter entries of one (row, state) per
+// thread: the 21 upper-triangle entries give the symmetric Voigt gradient, the inverse P = h0^-1
+// by cofactors (rows r1 x r2, r2 x r0, r0 x r1 of cof; P^T = cof / det), dE/dh0 =
+// -P^T (dE/dP) P^T + (dE/dw) w P^T with w = det / 8 (corner8) or det; states beyond the row's
+// count get zeros; thread (row, 0) also writes boltzmann dE/d(kT).
+__global__ void __launch_bounds__(256)
+param_grads_kernel(const double* __restrict__ entries, int64_t stride, int add_next,
+                   int64_t npar, int nrows, int max_states, const int32_t* __restrict__ nstates,
+                   const double* __restrict__ h0, int corner8, double boltzmann,
+                   double* __restrict__ g_cv, double* __restrict__ g_h0,
+                   double* __restrict__ g_free, double* __restrict__ g_teff) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows * max_states) return;
+  const int row = t / max_states, s = t % max_states;
+  const double* er = entries + (int64_t)row * npar * stride;
+  auto ent = [&](int64_t q) {
+    return add_next ? er[q * stride] + er[q * stride + 1] : er[q * stride];
+  };
+  const bool on = s < nstates[row];
+  const int64_t q0 = (int64_t)s * kNps;
+  double* gc = g_cv + (int64_t)t * 36;
+  int q = 0;
+  for (int I = 0; I < 6; ++I) {
+    for (int J = I; J < 6; ++J, ++q) {
+      const double v = on ? ent(q0 + q) : 0.0;
+      gc[6 * I + J] = v;
+      gc[6 * J + I] = v;
+    }
+  }
+  const double* h = h0 + (int64_t)t * 9;
+  double cof[9];
+  // cof row 0 = r1 x r2, row 1 = r2 x r0, row 2 = r0 x r1
+  for (int a = 0; a < 3; ++a) {
+    const double* u = h + 3 * ((a + 1) % 3);
+    const double* v = h + 3 * ((a + 2) % 3);
+    cof[3 * a + 0] = u[1] * v[2] - u[2] * v[1];
+    cof[3 * a + 1] = u[2] * v[0] - u[0] * v[2];
+    cof[3 * a + 2] = u[0] * v[1] - u[1] * v[0];
+  }
+  const double det = (h[0] * cof[0] + h[1] * cof[1]) + h[2] * cof[2];
+  double pt[9], gp[9], m1[9];
+  for (int k = 0; k < 9; ++k) {
+    pt[k] = cof[k] / det;
+    gp[k] = ent(q0 + 21 + k);
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      m1[3 * i + j] = (pt[3 * i] * gp[j] + pt[3 * i + 1] * gp[3 + j]) + pt[3 * i + 2] * gp[6 + j];
+  const double w = corner8 ? det / 8.0 : det;
+  const double gww = ent(q0 + 30) * w;
+  double* gh = g_h0 + (int64_t)t * 9;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double m2 =
+          (m1[3 * i] * pt[j] + m1[3 * i + 1] * pt[3 + j]) + m1[3 * i + 2] * pt[6 + j];
+      gh[3 * i + j] = on ? -m2 + gww * pt[3 * i + j] : 0.0;
+    }
+  g_free[t] = on ? ent(q0 + 31) : 0.0;
+  if (s == 0) g_teff[row] = ent((int64_t)max_states * kNps) * boltzmann;
+}
+
 struct mm_adjoint {
   mm_field* field = nullptr;
   int64_t npar = 0;            // parameter entries per row: max_states * 32 + 1
@@ -735,6 +796,30 @@ int mm_adjoint_vjp(mm_adjoint* a, const double* pos, const double* rvecs,
     (void)hipSetDevice(cur);
     if (e != hipSuccess) return hip_error(e, "vector-Jacobian position kernel");
   }
+  return MM_OK;
+}
+
+int mm_adjoint_param_grads(mm_adjoint* a, const double* entries, int64_t stride, int add_next,
+                           const double* h0, double boltzmann, double* g_cv, double* g_h0,
+                           double* g_free, double* g_teff, void* stream) {
+  if (!a) return set_error(MM_ERR_ARG, "adjoint is NULL");
+  if (!entries || !h0 || !g_cv || !g_h0 || !g_free || !g_teff)
+    return set_error(MM_ERR_ARG, "missing argument");
+  if (stride < 1 || (add_next && stride < 2))
+    return set_error(MM_ERR_ARG, "stride %lld with add_next %d", (long long)stride, add_next);
+  mm_field* f = a->field;
+  const int n = f->nrows * f->max_states;
+  if (n == 0) return MM_OK;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(f->device);
+  hipLaunchKernelGGL(param_grads_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, entries, stride, add_next, a->npar, f->nrows,
+                     f->max_states, f->nstates, h0, f->model == MM_MODEL_CORNER8 ? 1 : 0,
+                     boltzmann, g_cv, g_h0, g_free, g_teff);
+  const hipError_t e = hipGetLastError();
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess) return hip_error(e, "parameter gradient kernel");
   return MM_OK;
 }
 
