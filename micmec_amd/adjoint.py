@@ -448,7 +448,9 @@ def fit(problem, init, steps, lr_cv=1e-2, lr_h0=1e-4, callback=None, graph=True)
     graph: after three eager steps, one whole step (parameter table, forward force pass,
     loss, backward adjoint pass, Adam update) is captured in a CUDA graph and replayed for the
     remaining steps, so that the host launches one graph per step instead of some sixty
-    kernels; the steps are the same kernels on the same buffers either way."""
+    kernels; the steps are the same kernels on the same buffers either way. A problem whose
+    loss synchronises with the host or allocates outside torch cannot be captured: pass
+    graph=False for it."""
     cv0, h00 = init.cv.detach(), init.h0.detach()
     free, teff = init.free.detach(), init.teff.detach()
     a = torch.zeros_like(cv0, requires_grad=True)
