@@ -61,18 +61,28 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* smem
 // All 24 loads are issued before any arithmetic, and the shifts are formed with factors 0 or 1
 // (exact products) rather than data-dependent selects, so that no load ends up behind a branch
 // and a full wait; the sums are those of the selects, in the same order.
-// With kXmat every position is first taken through the pending transform xm (vecmat, the
-// product the scaling pass stores).
+// With kXmat every position is first taken through the pending transforms: xm2 (when not
+// null), then xm (vecmat, the products the scaling pass stores, in its order).
 template <bool kXmat = false>
 __device__ __forceinline__ void gather_vertices(const double* __restrict__ pos,
                                                 const double* __restrict__ rv, const int n[8],
                                                 int wrap, double r[8][3],
-                                                const double* __restrict__ xm = nullptr) {
+                                                const double* __restrict__ xm = nullptr,
+                                                const double* __restrict__ xm2 = nullptr) {
   double p[8][3];
 #pragma unroll
   for (int k = 0; k < 8; ++k)
 #pragma unroll
     for (int j = 0; j < 3; ++j) p[k][j] = pos[3 * (int64_t)n[k] + j];
+  if (kXmat && xm2) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      double t[3];
+      vecmat(p[k], xm2, t);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) p[k][j] = t[j];
+    }
+  }
   if (kXmat) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -117,8 +127,10 @@ __device__ constexpr int kUpperCorner[4] = {3, 5, 6, 7};
 // the single-state instance does not carry the registers of the state combination.
 // kVir: the virial as the sum of the corners' B^T S B (state_energy), which the vertex
 // gradients give as sum_v g_v (x) r_v; with kGrad false the pass then forms no gradient at all
-// (energy and virial only), and takes the node positions through the pending transform xmat
-// when that is given (a barostat post call whose position scaling waits).
+// (energy and virial only). The passes with gradients and the vertex virial, and those with the
+// corner virial and no gradients, take the node positions through the pending transforms xmat
+// (and xmat2 first, when its flag is set) when they are given: a barostat call whose position
+// scaling is stored by a later node pass.
 template <int kModel, bool kGrad, bool kMulti, bool kVir>
 __global__ void __launch_bounds__(kCellBlock, (kVir && !kGrad && !kMulti) ? 3 : 2)
 cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
@@ -128,7 +140,8 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
             const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
             int32_t max_states, int64_t W, const int32_t* __restrict__ cbase, int64_t S,
             double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks,
-            const double* __restrict__ xmat) {
+            const double* __restrict__ xmat, const double* __restrict__ xmat2,
+            const double* __restrict__ xmat2_on) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[(kMulti && !kGrad && !kVir) ? kMaxStates * kCellBlock : 1];
   // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
@@ -145,9 +158,11 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   double rv[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) rv[q] = rvb[q];
-  constexpr bool kCanX = kVir && !kGrad;
-  const double* xm =
-      (kCanX && xmat) ? xmat + (blk_sys ? (int64_t)blk_sys[bid] * rvecs_stride : 0) : nullptr;
+  constexpr bool kCanX = kVir != kGrad;
+  const int64_t soff = blk_sys ? (int64_t)blk_sys[bid] * rvecs_stride : 0;
+  const double* xm = (kCanX && xmat) ? xmat + soff : nullptr;
+  const double* xm2 =
+      (kCanX && xmat2 && xmat2_on[soff] != 0.0) ? xmat2 + soff : nullptr;
 
   const int64_t w = (int64_t)bid * kCellBlock + threadIdx.x;
   int n[8];
@@ -171,7 +186,7 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   if (valid) {
     double r[8][3];
     if (kCanX && xm)
-      gather_vertices<true>(pos, rv, n, wrap, r, xm);
+      gather_vertices<true>(pos, rv, n, wrap, r, xm, xm2);
     else
       gather_vertices(pos, rv, n, wrap, r);
     double ecell;
@@ -471,13 +486,16 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL((cell_kernel<MODEL, GRAD, MULTI, VIR>), dim3(NBL), dim3(kCellBlock), 0, S, \
                      a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap, a.blk_row, \
                      a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, a.gcell,  \
-                     a.partials, B0, nb, a.xmat)
+                     a.partials, B0, nb, a.xmat, a.xmat2, a.xmat2_on)
   // corner_virial: the virial from the corners; without records the pass then forms no
   // gradient at all
   const bool vir = a.corner_virial != 0;
   const bool kgrad = grad && !(vir && !a.store_records);
-  // only the pass without gradients takes the positions through a pending transform
-  if (a.xmat != nullptr && (kgrad || !vir)) return hipErrorInvalidValue;
+  // pending position transforms: the gradient pass with the vertex virial or the corner-virial
+  // pass without gradients, xmat with every use, xmat2 with its flag
+  if ((a.xmat2 != nullptr) != (a.xmat2_on != nullptr)) return hipErrorInvalidValue;
+  if (a.xmat2 != nullptr && a.xmat == nullptr) return hipErrorInvalidValue;
+  if (a.xmat != nullptr && kgrad == vir) return hipErrorInvalidValue;
   // The instance that takes longer is launched first, on the caller's stream: it then starts
   // right behind the preceding kernel instead of after the side stream's. That is the one
   // with more state evaluations (a multistate block counted as two), and the multistate one

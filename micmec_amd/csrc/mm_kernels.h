@@ -40,9 +40,14 @@ struct ForceArgs {
   int corner_virial = 0;    // 1: the virial as the corners' sum of B^T S B (no vertex gradient
                             //    needed for it)
   int store_records = 1;    // with corner_virial: 0 forms no gradient and stores no records
-  // with corner_virial and no records: a pending position transform [3][3] (row vectors, system s
-  // at xmat + s * rvecs_stride) that the pass applies to every node position it reads, or null
+  // pending position transforms [3][3] (row vectors, system s at xmat + s * rvecs_stride) that
+  // the pass applies to every node position it reads, xmat2 first when the flag at
+  // xmat2_on + s * rvecs_stride is set (both null: none): the position scaling of a barostat
+  // call that its own pass does not store. Only for the passes with gradients and the vertex
+  // virial, or with the corner virial and no gradients.
   const double* xmat = nullptr;
+  const double* xmat2 = nullptr;
+  const double* xmat2_on = nullptr;
   int split = -1;           // multistate blocks are [0, split), single-state [split, nb); -1: mixed
   hipStream_t side = nullptr;   // stream of the multistate instance (null: the caller's stream)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

@@ -258,7 +258,9 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // With kDrift instead, the first half kick and the drift of the step (verlet.py:141-146) with
 // the barostat's pre-call gradient, which nothing else reads: the pending velocity transforms
 // (and the Langevin thermostat's pre update, its ekin after the update in partial row 14),
-// then v += a dt / 2 and x += dt v, as the node pass kKickDrift does from a stored gpos.
+// then v += a dt / 2 and x += dt v, as the node pass kKickDrift does from a stored gpos, with
+// x first taken through the barostat call's pending position scaling (the previous post
+// call's when it still waits, then the pre call's), which its cell pass applied on the fly.
 // One node per thread (the gathers need every wave in flight that they can get); the node
 // blocks are dealt to the XCDs in contiguous ranges (xcd_block), and the partial sums are
 // stored per logical block.
@@ -335,12 +337,22 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
         langevin_thermo(S, n, 0, mass, vv);
         acc[0] += node_ekin(vv, mass);
       }
+      // the positions as the cell pass read them: the pending scaling(s) of the barostat call
+      double p[3], t[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) p[j] = x[3 * n + j];
+      if (S[MM_S_XMAT2_ON] != 0.0) {
+        vecmat(p, S + MM_S_XMAT2, t);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) p[j] = t[j];
+      }
+      vecmat(p, S + MM_S_XMAT, t);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double a = -g[j] / mass;
         vv[j] = vv[j] + (0.5 * a) * dt;
         v[3 * n + j] = vv[j];
-        x[3 * n + j] = x[3 * n + j] + dt * vv[j];
+        x[3 * n + j] = t[j] + dt * vv[j];
       }
     }
     }
@@ -963,10 +975,6 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       // (it is the kick-drift pass), so a NaN gradient shows here through the virial.
       take_force(S, f, true);
       take_nodes_after_vmat(S, nd);
-      // the scaling pass before this force applied a pending earlier position transform; its
-      // slot is cleared, entries included, as after steps taken one per call
-      S[MM_S_XMAT2_ON] = 0.0;
-      for (int q = 0; q < 9; ++q) S[MM_S_XMAT2 + q] = 0.0;
       update_baro_vel(S, sys, 0, 1);
       baro_call_end(S);
       if (baro_chain_on(S)) baro_chain_step(S);
@@ -981,6 +989,10 @@ __device__ void run_phase(int phase, double* S, const double (&f)[10],
       break;
     }
     case kPhMainEndNpt: {
+      // gather_drift has stored the positions through a pending earlier position transform; its
+      // slot is cleared, entries included, as after steps taken one per call
+      S[MM_S_XMAT2_ON] = 0.0;
+      for (int q = 0; q < 9; ++q) S[MM_S_XMAT2 + q] = 0.0;
       clear_vtransforms(S);
       take_force(S, f, true);
       const double ekin0 = S[MM_S_EKIN];
@@ -1302,17 +1314,23 @@ hipError_t gather_drift(mm_md* md, bool lt, hipStream_t s) {
 }
 
 // The barostat call MTKBarostat.baro after its first scalar phase: positions scaled, force,
-// then the second barostat-velocity update. The pre call (pre_call) leaves its records for
-// gather_drift, which applies its velocity transform with the kick; the post call's transform
-// is applied by its gather (scale_v) or stays pending. defer_x (a post call without records):
-// the positions are not scaled in a pass of their own; the cell pass takes them through the
-// transform as it reads them, and the next scaling pass applies it before its own.
+// then the second barostat-velocity update. The pre call (pre_call) scales no positions in a
+// pass of its own: its cell pass reads them through the pending transforms, and it leaves its
+// records for gather_drift, which stores the scaled positions and applies the velocity
+// transform with the kick. The post call's velocity transform is applied by its gather
+// (scale_v) or stays pending; with defer_x (a post call without records) its position scaling
+// also waits (the cell pass applies it on the fly, the next gather_drift stores it).
 hipError_t baro_call(mm_md* md, int end_phase, hipStream_t s, bool scale_v,
                      int gmode = kGatherStore, bool corner_virial = false,
                      bool pre_call = false, bool defer_x = false) {
-  if (!defer_x) MM_TRY(node(md, kScalePos, s));
+  if (!defer_x && !pre_call) MM_TRY(node(md, kScalePos, s));
   if (pre_call) {
+    // the pre call's scaling (after a pending post-call one) is applied by its cell pass as it
+    // reads the positions and stored by gather_drift
     ForceArgs a = force_args(md);
+    a.xmat = md->S + MM_S_XMAT;
+    a.xmat2 = md->S + MM_S_XMAT2;
+    a.xmat2_on = md->S + MM_S_XMAT2_ON;
     MM_TRY(launch_cells(a, s));
   } else {
     MM_TRY(force(md, false, false, s, scale_v, gmode, corner_virial,
