@@ -11,7 +11,10 @@ resident on the GPU before the timed region starts.
 Prints one JSON line. ``roofline`` refers to the dominant kernel (the per-cell force pass),
 timed with HIP events on the stream it is launched on; ``cpu_baseline`` times the reference's
 algorithm restated as a literal per-cell loop on one host core, on a bounded sample of the same
-workload (an 8^3 grid), with the C restatement on all host cores beside it.
+workload (an 8^3 grid), with the C restatement on all host cores beside it. Further legs
+report the other workloads: ``batched_replicas`` (512 uneven replicas as one batch and one at
+a time), ``nvt_nhc32`` (32^3 NVT with a Nose-Hoover chain, 2000 steps in one call),
+``cg_relaxation`` (CG at 128^3) and ``differentiation`` (the 64 x 16^3 Adam fit).
 """
 
 import argparse
@@ -57,6 +60,9 @@ def parse():
     ap.add_argument("--cg", dest="cg", action="store_true", default=True)
     ap.add_argument("--no-cg", dest="cg", action="store_false")
     ap.add_argument("--cg-steps", type=int, default=200)
+    ap.add_argument("--nvt", dest="nvt", action="store_true", default=True)
+    ap.add_argument("--no-nvt", dest="nvt", action="store_false")
+    ap.add_argument("--nvt-steps", type=int, default=2000)
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="launch only the cell pass this many times, then copy 1 GiB with a "
                          "plain copy kernel (the counter calibration), and exit")
@@ -280,6 +286,36 @@ def pmc_probe(n, model, launches):
                                     "copy_bytes_each_way": a.numel() * 8, "copies": 3}}))
 
 
+def nvt_nhc32(steps):
+    """The mid-size plain workload: 32^3 cells, 20% type B, NVT with a Nose-Hoover chain of 3
+    at 300 K (timecon 100 fs), dt 1 fs, the steps of one verlet.run call."""
+    import numpy as np
+    import torch
+    from micmec_amd import units
+    from micmec_amd.pes import ForcePartMechanical, MicMecForceField
+    from micmec_amd.sampling import NHCThermostat, VerletIntegrator
+    from micmec_amd.workloads import make_system, workload
+    w = workload("nhc32", steps=steps)
+    system = make_system(w["shape"], p_b=w["p_b"], p_hole=w["p_hole"], seed=w["seed"])
+    mmf = MicMecForceField(system, [ForcePartMechanical(system)])
+    np.random.seed(w["seed"])
+    fs = units.femtosecond
+    thermo = NHCThermostat(temp=w["temp"], timecon=w["timecon_fs"] * fs, chainlength=w["chain"])
+    verlet = VerletIntegrator(mmf, timestep=w["timestep_fs"] * fs, hooks=[thermo],
+                              temp0=w["temp"])
+    verlet.run(50)
+    e0, k0 = verlet.econs, verlet.ekin
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    verlet.run(w["steps"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": "nhc32", "grid": list(w["shape"]), "cells": system.ncells,
+            "steps": w["steps"], "ms_per_step": dt / w["steps"] * 1e3,
+            "cell_steps_per_s": system.ncells * w["steps"] / dt,
+            "econs_drift_over_ekin": abs(verlet.econs - e0) / k0}
+
+
 def cg_relaxation(n, steps, model):
     """CG geometry optimisation of the 128^3 system from a 2% random strain (positions only,
     domain fixed): `steps` iterations, thresholds tight enough that all of them run."""
@@ -405,6 +441,9 @@ def main():
         print("batched replicas ...", file=sys.stderr, flush=True)
         out["batched_replicas"] = batched_vs_sequential(args.replicas, args.batched_steps,
                                                         args.sequential_steps)
+    if args.nvt:
+        print("NVT nhc32 ...", file=sys.stderr, flush=True)
+        out["nvt_nhc32"] = nvt_nhc32(args.nvt_steps)
     if args.cg:
         print("CG relaxation ...", file=sys.stderr, flush=True)
         out["cg_relaxation"] = cg_relaxation(n, args.cg_steps, args.model)
