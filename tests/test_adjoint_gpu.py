@@ -158,6 +158,37 @@ def param_fd(field, pos, rv, params, weights):
     return fd_cv, fd_h0, fd_free, fd_teff
 
 
+def test_parameter_gradient_kernel_sums_value_and_tangent_rows(setup):
+    """mm_adjoint_param_grads with the raw [nrows, npar, 2] rows of a dual launch (stride 2,
+    add_next) equals the same kernel on the summed rows (stride 1), bit for bit; the absent
+    states get zeros and the C entry point refuses a stride that cannot hold both rows."""
+    import ctypes
+    from micmec_amd import _native
+    field, pos, rv, params = setup
+    gen = torch.Generator().manual_seed(11)
+    dpos = (torch.randn(pos.shape, generator=gen, dtype=torch.float64) * 0.01 * A).to(pos.device)
+    drv = (torch.randn(rv.shape, generator=gen, dtype=torch.float64) * 0.01 * A).to(pos.device)
+    out = field.adjoint(pos, rv, dpos.contiguous(), drv.contiguous())
+    raw = torch.stack([out["params"], out["dparams"]], -1).contiguous()
+    fused = field.param_grads(raw, params, add_next=True)
+    summed = field.param_grads((out["params"] + out["dparams"]).contiguous(), params)
+    for a, b in zip(fused, summed):
+        assert torch.equal(a, b)
+    for t in range(field.nrows):
+        ns = int(params.nstates[t])
+        assert not fused[0][t, ns:].any() and not fused[1][t, ns:].any()
+        assert not fused[2][t, ns:].any()
+    kw = dict(dtype=torch.float64, device=pos.device)
+    T, S = field.nrows, field.max_states
+    bufs = [torch.empty((T, S, 36), **kw), torch.empty((T, S, 9), **kw),
+            torch.empty((T, S), **kw), torch.empty((T,), **kw)]
+    h0 = params.h0.detach().contiguous()
+    rc = field.lib.mm_adjoint_param_grads(field._adj, _native.ptr(raw), 1, 1, _native.ptr(h0),
+                                          1.0, *[_native.ptr(b) for b in bufs],
+                                          _native.stream_ptr())
+    assert rc == _native.MM_ERR_ARG
+
+
 def test_parameter_gradients_fd(setup):
     field, pos, rv, params = setup
     weights = torch.tensor([1.0, -0.5, 2.0], dtype=torch.float64)
