@@ -472,9 +472,16 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   const bool part = a.split >= 0;
   const int m0 = 0, mn = part ? a.split : nb;
   const int s0 = part ? a.split : 0, sn = part ? nb - a.split : nb;
-  // the second instance runs on the side stream, concurrently with the first
+  // The second instance runs on the side stream, concurrently with the first, when the pass
+  // has more cell blocks than the device holds at once (256 CUs x 2 blocks of 256 lanes at two
+  // waves a SIMD); a smaller pass takes the time of one block per instance, less than the
+  // fork and join cost, so both instances then run back to back on the caller's stream:
+  // NVT-NHC / NPT step at 32^3 (129 blocks) 51.0 / 120.7 -> 47.8 / 109 us, at 48^3 (about
+  // 430 blocks) 59.2 / 138.7 -> 58.8 / 127.5 us; at 64^3 (about 1030 blocks) the fork is the
+  // faster (NVT 74.0 vs 76.4 us), at 96^3 too (NPT 329 vs 343 us). The results are the same
+  // bits either way.
   const bool fork = a.side != nullptr && a.ev_fork != nullptr && a.ev_join != nullptr &&
-                    mn > 0 && sn > 0;
+                    mn > 0 && sn > 0 && nb > kForkMinBlocks;
   hipStream_t ms = fork ? a.side : stream;
   hipError_t e = hipSuccess;
   if (fork) {

@@ -7,6 +7,8 @@
 namespace mm {
 
 constexpr int kCellBlock = 256;
+// Cell passes of more blocks than this run their two instances concurrently.
+constexpr int kForkMinBlocks = 512;
 constexpr int kPartialStride = 16;
 constexpr int kMaxStates = 8;  // metastable states per cell type
 
