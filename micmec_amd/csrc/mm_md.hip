@@ -144,31 +144,69 @@ __device__ __forceinline__ double node_ekin(const double v[3], double m) {
 
 // Pending velocity transforms, applied in the order the reference performs them:
 // factor A (thermostat post of the previous half), matrix M (barostat), matrix M2 (the next
-// barostat call, when M still waits), factor B (thermostat pre).
-__device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, double v[3]) {
+// barostat call, when M still waits), factor B (thermostat pre). A null pointer is a transform
+// that is off.
+__device__ __forceinline__ void apply_vtransform(const double* fa, const double* M,
+                                                 const double* M2, const double* fb,
+                                                 double v[3]) {
 #pragma clang fp contract(off)
-  if (S[MM_S_VSCALE_A_ON] != 0.0) {
-    const double f = S[MM_S_VSCALE_A];
+  if (fa) {
+    const double f = *fa;
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = v[j] * f;
   }
-  if (S[MM_S_VMAT_ON] != 0.0) {
+  if (M) {
     double t[3];
-    vecmat(v, S + MM_S_VMAT, t);
+    vecmat(v, M, t);
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = t[j];
   }
-  if (S[MM_S_VMAT2_ON] != 0.0) {
+  if (M2) {
     double t[3];
-    vecmat(v, S + MM_S_VMAT2, t);
+    vecmat(v, M2, t);
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = t[j];
   }
-  if (S[MM_S_VSCALE_B_ON] != 0.0) {
-    const double f = S[MM_S_VSCALE_B];
+  if (fb) {
+    const double f = *fb;
 #pragma unroll
     for (int j = 0; j < 3; ++j) v[j] = v[j] * f;
   }
+}
+
+// The transforms as the scalar block holds them, read where they are applied.
+__device__ __forceinline__ void apply_vtransform(const double* __restrict__ S, double v[3]) {
+  apply_vtransform(S[MM_S_VSCALE_A_ON] != 0.0 ? S + MM_S_VSCALE_A : nullptr,
+                   S[MM_S_VMAT_ON] != 0.0 ? S + MM_S_VMAT : nullptr,
+                   S[MM_S_VMAT2_ON] != 0.0 ? S + MM_S_VMAT2 : nullptr,
+                   S[MM_S_VSCALE_B_ON] != 0.0 ? S + MM_S_VSCALE_B : nullptr, v);
+}
+
+// The transforms read from the scalar block once, for a pass over several nodes a lane.
+struct VTransform {
+  bool a_on, m_on, m2_on, b_on;
+  double fa, fb, m[9], m2[9];
+};
+
+__device__ __forceinline__ VTransform load_vtransform(const double* __restrict__ S) {
+  VTransform T;
+  T.a_on = S[MM_S_VSCALE_A_ON] != 0.0;
+  T.m_on = S[MM_S_VMAT_ON] != 0.0;
+  T.m2_on = S[MM_S_VMAT2_ON] != 0.0;
+  T.b_on = S[MM_S_VSCALE_B_ON] != 0.0;
+  T.fa = T.a_on ? S[MM_S_VSCALE_A] : 1.0;
+  T.fb = T.b_on ? S[MM_S_VSCALE_B] : 1.0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    T.m[q] = T.m_on ? S[MM_S_VMAT + q] : 0.0;
+    T.m2[q] = T.m2_on ? S[MM_S_VMAT2 + q] : 0.0;
+  }
+  return T;
+}
+
+__device__ __forceinline__ void apply_vtransform(const VTransform& T, double v[3]) {
+  apply_vtransform(T.a_on ? &T.fa : nullptr, T.m_on ? T.m : nullptr, T.m2_on ? T.m2 : nullptr,
+                   T.b_on ? &T.fb : nullptr, v);
 }
 
 template <int kMode>
@@ -183,28 +221,55 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 #pragma unroll
   for (int q = 0; q < 13; ++q) acc[q] = 0.0;
   double lt[1] = {0.0};
+  constexpr int kIt = kNodeBlock / kNodeThreads;
+  const int64_t n0 = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+  if (kMode == kKickDrift || kMode == kKickDriftLT) {
+    // every load of the lane's nodes first, then the arithmetic and the stores, so that the
+    // loads are in flight together (a pass over a small system is then one memory round trip
+    // instead of two a node)
+    const double dt = S[MM_S_TIMESTEP];
+    const VTransform T = load_vtransform(S);
+    double mass[kIt], vv[kIt][3], gg[kIt][3], xx[kIt][3];
 #pragma unroll
-  for (int it = 0; it < kNodeBlock / kNodeThreads; ++it) {
-    const int64_t n = (int64_t)blockIdx.x * kNodeBlock + it * kNodeThreads + threadIdx.x;
-    if (n >= N) break;
-    if (kMode == kKickDrift || kMode == kKickDriftLT) {
-      const double dt = S[MM_S_TIMESTEP];
-      const double mass = m[n];
-      double vv[3];
+    for (int it = 0; it < kIt; ++it) {
+      const int64_t n = n0 + it * kNodeThreads;
+      mass[it] = 1.0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
-      apply_vtransform(S, vv);
+      for (int j = 0; j < 3; ++j) vv[it][j] = gg[it][j] = xx[it][j] = 0.0;
+      if (n < N) {
+        mass[it] = m[n];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          vv[it][j] = v[3 * n + j];
+          gg[it][j] = g[3 * n + j];
+          xx[it][j] = x[3 * n + j];
+        }
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int64_t n = n0 + it * kNodeThreads;
+      if (n >= N) break;
+      apply_vtransform(T, vv[it]);
       if (kMode == kKickDriftLT) {
-        langevin_thermo(S, n, 0, mass, vv);
-        lt[0] += node_ekin(vv, mass);
+        langevin_thermo(S, n, 0, mass[it], vv[it]);
+        lt[0] += node_ekin(vv[it], mass[it]);
       }
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const double a = -g[3 * n + j] / mass;
-        vv[j] = vv[j] + (0.5 * a) * dt;
-        v[3 * n + j] = vv[j];
-        x[3 * n + j] = x[3 * n + j] + dt * vv[j];
+        const double a = -gg[it][j] / mass[it];
+        vv[it][j] = vv[it][j] + (0.5 * a) * dt;
+        v[3 * n + j] = vv[it][j];
+        x[3 * n + j] = xx[it][j] + dt * vv[it][j];
       }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int64_t n = n0 + it * kNodeThreads;
+    if (n >= N) break;
+    if (kMode == kKickDrift || kMode == kKickDriftLT) {
+      // done above
     } else if (kMode == kScalePos) {
       double p[3], t[3];
 #pragma unroll
