@@ -219,6 +219,55 @@ __device__ __forceinline__ void vecmat(const double v[3], const double* M, doubl
   for (int j = 0; j < 3; ++j) out[j] = (v[0] * M[j] + v[1] * M[3 + j]) + v[2] * M[6 + j];
 }
 
+// g = the sum of node n's records in k order (mmff.py:303-318): record e = ncell[k * N + n]
+// (none when e < 0) holds the three components gcell[(3 * (e & (2^rshift - 1)) + j) * RS +
+// (e >> rshift)]; g starts at +0.0 and takes the records present in k order.
+// kAllLoads: the eight indices are loaded first and then all 24 components, a none reading those
+// of record 0 (always allocated) and left out of the sum, so that every load of the lane is in
+// flight at once (two memory round trips instead of up to sixteen), at the cost of about 50
+// more VGPRs; the same bits as skipping the nones. Without it each record is loaded behind
+// the test of its index, which keeps the registers few: the form for passes with enough
+// waves to hide the latency.
+template <bool kAllLoads>
+__device__ __forceinline__ void sum_node_records(const int32_t* __restrict__ ncell,
+                                                 const double* __restrict__ gcell, int64_t N,
+                                                 int64_t RS, int rshift, int64_t n,
+                                                 double g[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) g[j] = 0.0;
+  if (!kAllLoads) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
+      if (e >= 0) {
+        const int64_t w = e >> rshift;
+        const int p = e & ((1 << rshift) - 1);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
+      }
+    }
+    return;
+  }
+  int32_t e[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) e[k] = __builtin_nontemporal_load(&ncell[k * N + n]);
+  double r[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int32_t c = e[k] >= 0 ? e[k] : 0;
+    const int64_t w = c >> rshift;
+    const int p = c & ((1 << rshift) - 1);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r[k][j] = __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[j] = e[k] >= 0 ? g[j] + r[k][j] : g[j];
+}
+
 // Workgroups are dealt to the 8 XCDs of the device in turn (block b runs on XCD b mod 8), and
 // every XCD has its own L2. Neighbouring blocks share cache lines (the cell segments share
 // nodes, the node blocks' record gathers share the lines at their ends), so each XCD is given

@@ -360,7 +360,7 @@ __device__ __forceinline__ void block_sum_max_store(double (&v)[NS + NM], double
 // DOT (a line-search trial), also the workgroup partials of sum_n gpos_n . dir_n,
 // sum_n |gpos_n|^2, sum_n gpos_n . gold_n and max_n |gpos_n|^2 into dpart rows 0-3 (row stride =
 // workgroups).
-template <bool DOT>
+template <bool DOT, bool kAllLoads>
 __global__ void __launch_bounds__(256)
 gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell, int64_t N,
               int64_t RS, int rshift, double* __restrict__ gpos, int accumulate, int* __restrict__ nanflag,
@@ -386,19 +386,7 @@ gather_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcel
     }
   }
   double g[3] = {0.0, 0.0, 0.0};
-  if (n < N) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
-      if (e >= 0) {
-        const int64_t w = e >> rshift;
-        const int p = e & ((1 << rshift) - 1);
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
-      }
-    }
-  }
+  if (n < N) sum_node_records<kAllLoads>(ncell, gcell, N, RS, rshift, n, g);
   __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (n < N) {
@@ -544,8 +532,14 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
 hipError_t launch_gather(const ForceArgs& a, hipStream_t stream) {
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
-  hipLaunchKernelGGL(gather_kernel<false>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                     a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag, nullptr, nullptr, nullptr);
+  if (gb <= kGatherAllLoadsMaxBlocks)
+    hipLaunchKernelGGL((gather_kernel<false, true>), dim3(gb), dim3(256), 0, stream, a.ncell,
+                       a.gcell, a.N, a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag,
+                       nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((gather_kernel<false, false>), dim3(gb), dim3(256), 0, stream, a.ncell,
+                       a.gcell, a.N, a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag,
+                       nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -553,8 +547,14 @@ hipError_t launch_gather_dot(const ForceArgs& a, const double* dir, const double
                              double* dpart, hipStream_t stream) {
   if (a.gpos == nullptr || a.N <= 0 || a.W <= 0) return hipSuccess;
   const int gb = (int)((a.N + 255) / 256);
-  hipLaunchKernelGGL(gather_kernel<true>, dim3(gb), dim3(256), 0, stream, a.ncell, a.gcell, a.N,
-                     a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag, dir, gold, dpart);
+  if (gb <= kGatherAllLoadsMaxBlocks)
+    hipLaunchKernelGGL((gather_kernel<true, true>), dim3(gb), dim3(256), 0, stream, a.ncell,
+                       a.gcell, a.N, a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag, dir,
+                       gold, dpart);
+  else
+    hipLaunchKernelGGL((gather_kernel<true, false>), dim3(gb), dim3(256), 0, stream, a.ncell,
+                       a.gcell, a.N, a.rstride, a.rshift, a.gpos, a.accumulate, a.nanflag, dir,
+                       gold, dpart);
   return hipGetLastError();
 }
 

@@ -9,6 +9,13 @@ namespace mm {
 constexpr int kCellBlock = 256;
 // Cell passes of more blocks than this run their two instances concurrently.
 constexpr int kForkMinBlocks = 512;
+// Node gathers over at most this many 256-node blocks (at most four waves a SIMD) load all of
+// a node's records at once (sum_node_records<true>): latency bounds such a pass. NVT-NHC / NPT
+// step, records one at a time -> all at once: 32^3 46.3 / 105.9 -> 43.9 / 103.4 us, 48^3
+// 57.5 / 126.5 -> 54.1 / 121.6 us, 64^3 74.4 / 170.4 -> 73.3 / 170.9 us; with more blocks the
+// extra registers cost more than the loads in flight gain (80^3 NPT 223 -> 234 us, 128^3
+// 650 -> 666 us). The same bits either way.
+constexpr int kGatherAllLoadsMaxBlocks = 1024;
 constexpr int kPartialStride = 16;
 constexpr int kMaxStates = 8;  // metastable states per cell type
 

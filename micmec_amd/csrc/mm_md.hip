@@ -329,7 +329,7 @@ node_kernel(double* __restrict__ x, double* __restrict__ v, const double* __rest
 // One node per thread (the gathers need every wave in flight that they can get); the node
 // blocks are dealt to the XCDs in contiguous ranges (xcd_block), and the partial sums are
 // stored per logical block.
-template <bool kKick, bool kLT, bool kScaleV = false, bool kDrift = false>
+template <bool kKick, bool kLT, bool kScaleV, bool kDrift, bool kAllLoads>
 __global__ void __launch_bounds__(kNodeBlock)
 gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__ gcell,
                    int64_t N, int64_t RS, double* __restrict__ gpos, double* __restrict__ v,
@@ -345,34 +345,39 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
   {
     const int64_t n = (int64_t)blk * kNodeBlock + threadIdx.x;
     if (n < N) {
+    // the node's own values: with kAllLoads loaded before the records, so that they arrive
+    // with them
+    double mass = 1.0, v0[3] = {0.0, 0.0, 0.0}, x0[3] = {0.0, 0.0, 0.0};
+    auto load_node = [&]() {
+      if (kKick || kDrift) mass = m[n];
+      if (kKick || kDrift || kScaleV) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v0[j] = v[3 * n + j];
+      }
+      if (kDrift) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) x0[j] = x[3 * n + j];
+      }
+    };
+    if (kAllLoads) load_node();
     double g[3] = {0.0, 0.0, 0.0};
     if (gmode != kGatherNone) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int32_t e = __builtin_nontemporal_load(&ncell[k * N + n]);
-        if (e >= 0) {
-          const int64_t w = e >> 2;
-          const int p = e & 3;
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            g[j] += __builtin_nontemporal_load(&gcell[(int64_t)(3 * p + j) * RS + w]);
-        }
-      }
+      sum_node_records<kAllLoads>(ncell, gcell, N, RS, 2, n, g);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         if (gmode == kGatherStore) gpos[3 * n + j] = g[j];
         if (g[j] != g[j]) acc[12] = 1.0;
       }
     }
+    if (!kAllLoads) load_node();
     if (kKick) {
 #pragma clang fp contract(off)
       const double dt = S[MM_S_TIMESTEP];
-      const double mass = m[n];
       double vv[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double a = -g[j] / mass;
-        vv[j] = v[3 * n + j] + (0.5 * a) * dt;
+        vv[j] = v0[j] + (0.5 * a) * dt;
       }
       if (kLT) {
         acc[NA - 1] += node_ekin(vv, mass);
@@ -385,7 +390,7 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     if (kScaleV) {
       double vv[3];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
+      for (int j = 0; j < 3; ++j) vv[j] = v0[j];
       apply_vtransform(S, vv);
 #pragma unroll
       for (int j = 0; j < 3; ++j) v[3 * n + j] = vv[j];
@@ -393,10 +398,9 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
     if (kDrift) {
 #pragma clang fp contract(off)
       const double dt = S[MM_S_TIMESTEP];
-      const double mass = m[n];
       double vv[3];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) vv[j] = v[3 * n + j];
+      for (int j = 0; j < 3; ++j) vv[j] = v0[j];
       apply_vtransform(S, vv);
       if (kLT) {
         langevin_thermo(S, n, 0, mass, vv);
@@ -405,7 +409,7 @@ gather_kick_kernel(const int32_t* __restrict__ ncell, const double* __restrict__
       // the positions as the cell pass read them: the pending scaling(s) of the barostat call
       double p[3], t[3];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) p[j] = x[3 * n + j];
+      for (int j = 0; j < 3; ++j) p[j] = x0[j];
       if (S[MM_S_XMAT2_ON] != 0.0) {
         vecmat(p, S + MM_S_XMAT2, t);
 #pragma unroll
@@ -1335,10 +1339,18 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
   if (e != hipSuccess) return e;
   // with no records, no transform to apply and no kick, the evaluation has no node pass
   if (md->N > 0 && !(gmode == kGatherNone && !scale_v && !kick)) {
-#define MM_GATHER(K, T, V)                                                                   \
-  hipLaunchKernelGGL((gather_kick_kernel<K, T, V>), dim3(md->nbn), dim3(kNodeBlock), 0, s,    \
-                     a.ncell, a.gcell, md->N, a.rstride, md->gpos, md->vel, md->masses, md->S,      \
-                     md->node_blk_sys, md->npart, gmode)
+    const bool all = md->nbn <= kGatherAllLoadsMaxBlocks;
+#define MM_GATHER1(K, T, V, A)                                                               \
+  hipLaunchKernelGGL((gather_kick_kernel<K, T, V, false, A>), dim3(md->nbn), dim3(kNodeBlock), \
+                     0, s, a.ncell, a.gcell, md->N, a.rstride, md->gpos, md->vel, md->masses,  \
+                     md->S, md->node_blk_sys, md->npart, gmode)
+#define MM_GATHER(K, T, V)          \
+  do {                              \
+    if (all)                        \
+      MM_GATHER1(K, T, V, true);    \
+    else                            \
+      MM_GATHER1(K, T, V, false);   \
+  } while (0)
     if (kick && lt)
       MM_GATHER(true, true, false);
     else if (kick)
@@ -1348,6 +1360,7 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
     else
       MM_GATHER(false, false, false);
 #undef MM_GATHER
+#undef MM_GATHER1
     e = hipGetLastError();
   }
   return e;
@@ -1365,15 +1378,20 @@ hipError_t force(mm_md* md, bool kick, bool lt, hipStream_t s, bool scale_v = fa
 hipError_t gather_drift(mm_md* md, bool lt, hipStream_t s) {
   if (md->N == 0) return hipSuccess;
   mm_field* f = md->field;
-#define MM_DRIFT(T)                                                                          \
-  hipLaunchKernelGGL((gather_kick_kernel<false, T, false, true>), dim3(md->nbn),             \
+  const bool all = md->nbn <= kGatherAllLoadsMaxBlocks;
+#define MM_DRIFT(T, A)                                                                       \
+  hipLaunchKernelGGL((gather_kick_kernel<false, T, false, true, A>), dim3(md->nbn),          \
                      dim3(kNodeBlock), 0, s, f->nrec, f->gcell, md->N, f->S, md->gpos,       \
                      md->vel, md->masses, md->S, md->node_blk_sys, md->npart, kGatherKeep,   \
                      md->pos)
-  if (lt)
-    MM_DRIFT(true);
+  if (lt && all)
+    MM_DRIFT(true, true);
+  else if (lt)
+    MM_DRIFT(true, false);
+  else if (all)
+    MM_DRIFT(false, true);
   else
-    MM_DRIFT(false);
+    MM_DRIFT(false, false);
 #undef MM_DRIFT
   return hipGetLastError();
 }
