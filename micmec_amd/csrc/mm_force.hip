@@ -131,22 +131,26 @@ __device__ constexpr int kUpperCorner[4] = {3, 5, 6, 7};
 // corner virial and no gradients, take the node positions through the pending transforms xmat
 // (and xmat2 first, when its flag is set) when they are given: a barostat call whose position
 // scaling is stored by a later node pass.
+#define MM_CELL_PARAMS                                                                        \
+  const double *__restrict__ pos, const double *__restrict__ rvecs, int64_t rvecs_stride,     \
+      const int32_t *__restrict__ blk_sys, const int32_t *__restrict__ cnode,                 \
+      const int32_t *__restrict__ cwrap, const int32_t *__restrict__ blk_row,                 \
+      const double *__restrict__ tab_g, const double *__restrict__ kt_g,                      \
+      const int32_t *__restrict__ nstates_g, int32_t max_states, int64_t W,                   \
+      const int32_t *__restrict__ cbase, int64_t S, double *__restrict__ gcell,               \
+      double *__restrict__ partials, int blk0, int nblocks, const double *__restrict__ xmat,  \
+      const double *__restrict__ xmat2, const double *__restrict__ xmat2_on
+#define MM_CELL_ARGS                                                                          \
+  pos, rvecs, rvecs_stride, blk_sys, cnode, cwrap, blk_row, tab_g, kt_g, nstates_g,            \
+      max_states, W, cbase, S, gcell, partials, blk0, nblocks, xmat, xmat2, xmat2_on
+
+// The work of cell block bid (see cell_kernel).
 template <int kModel, bool kGrad, bool kMulti, bool kVir>
-__global__ void __launch_bounds__(kCellBlock, (kVir && !kGrad && !kMulti) ? 3 : 2)
-cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
-            int64_t rvecs_stride, const int32_t* __restrict__ blk_sys,
-            const int32_t* __restrict__ cnode, const int32_t* __restrict__ cwrap,
-            const int32_t* __restrict__ blk_row, const double* __restrict__ tab_g,
-            const double* __restrict__ kt_g, const int32_t* __restrict__ nstates_g,
-            int32_t max_states, int64_t W, const int32_t* __restrict__ cbase, int64_t S,
-            double* __restrict__ gcell, double* __restrict__ partials, int blk0, int nblocks,
-            const double* __restrict__ xmat, const double* __restrict__ xmat2,
-            const double* __restrict__ xmat2_on) {
+__device__ __forceinline__ void cell_body(int bid, MM_CELL_PARAMS) {
   __shared__ double red[10 * (kCellBlock / 64)];
   __shared__ double es_sh[(kMulti && !kGrad && !kVir) ? kMaxStates * kCellBlock : 1];
   // running weighted gradient sum of the one-pass multistate gradient: [24][kCellBlock]
   __shared__ double gs_sh[(kMulti && kGrad) ? 24 * kCellBlock : 1];
-  const int bid = blk0 + xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int trow = blk_row[bid];
   // a block of the other kind (full-range launches only) leaves its partials to the other
   // instance
@@ -320,6 +324,27 @@ cell_kernel(const double* __restrict__ pos, const double* __restrict__ rvecs,
   block_reduce_store<10, kCellBlock>(acc, red, partials + bid, nblocks);
 }
 
+template <int kModel, bool kGrad, bool kMulti, bool kVir>
+__global__ void __launch_bounds__(kCellBlock, (kVir && !kGrad && !kMulti) ? 3 : 2)
+cell_kernel(MM_CELL_PARAMS) {
+  cell_body<kModel, kGrad, kMulti, kVir>(blk0 + xcd_block((int)blockIdx.x, (int)gridDim.x),
+                                         MM_CELL_ARGS);
+}
+
+// Both kinds of block in one launch, each block running the body of its own kind: for a pass
+// that the device holds at once, where the two instances would otherwise run one after the
+// other, each taking about one block's time.
+template <int kModel, bool kGrad, bool kVir>
+__global__ void __launch_bounds__(kCellBlock, 2) cell_kernel_both(MM_CELL_PARAMS) {
+  const int bid = blk0 + xcd_block((int)blockIdx.x, (int)gridDim.x);
+  if (nstates_g[blk_row[bid]] > 1)
+    cell_body<kModel, kGrad, true, kVir>(bid, MM_CELL_ARGS);
+  else
+    cell_body<kModel, kGrad, false, kVir>(bid, MM_CELL_ARGS);
+}
+#undef MM_CELL_ARGS
+#undef MM_CELL_PARAMS
+
 // NaN-propagating maximum (the result is NaN when either operand is).
 __device__ __forceinline__ double nan_max(double a, double b) {
   return (a > b || a != a) ? a : b;
@@ -462,12 +487,13 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   const int s0 = part ? a.split : 0, sn = part ? nb - a.split : nb;
   // The second instance runs on the side stream, concurrently with the first, when the pass
   // has more cell blocks than the device holds at once (256 CUs x 2 blocks of 256 lanes at two
-  // waves a SIMD); a smaller pass takes the time of one block per instance, less than the
-  // fork and join cost, so both instances then run back to back on the caller's stream:
-  // NVT-NHC / NPT step at 32^3 (129 blocks) 51.0 / 120.7 -> 47.8 / 109 us, at 48^3 (about
-  // 430 blocks) 59.2 / 138.7 -> 58.8 / 127.5 us; at 64^3 (about 1030 blocks) the fork is the
-  // faster (NVT 74.0 vs 76.4 us), at 96^3 too (NPT 329 vs 343 us). The results are the same
-  // bits either way.
+  // waves a SIMD). A smaller pass takes the time of one block per instance, less than the fork
+  // and join cost: it runs both kinds of block in one launch (cell_kernel_both). NVT-NHC / NPT
+  // step at 32^3 (129 blocks), fork -> back to back on one stream -> one launch: 51.0 / 120.7
+  // -> 47.8 / 109 -> 37.4 / 81.1 us (the last two with the gathers' all-loads form), at 48^3
+  // (about 430 blocks) 59.2 / 138.7 -> 58.8 / 127.5 us, -> 53.8 / 117.5 us in one launch; at
+  // 64^3 (about 1030 blocks) the fork is the faster (NVT 74.0 vs 76.4 us back to back), at
+  // 96^3 too (NPT 329 vs 343 us). The results are the same bits in every form.
   const bool fork = a.side != nullptr && a.ev_fork != nullptr && a.ev_join != nullptr &&
                     mn > 0 && sn > 0 && nb > kForkMinBlocks;
   hipStream_t ms = fork ? a.side : stream;
@@ -497,9 +523,18 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   // without gradients and with the corner virial, where the single-state instance runs three
   // waves a SIMD and the multistate one two.
   const bool multi_lead = (vir && !kgrad) || 2 * (int64_t)mn > (int64_t)sn;
+  // a pass that the device holds at once, with blocks of both kinds: one launch
+  const bool both = mn > 0 && sn > 0 && nb <= kForkMinBlocks;
+#define MM_LAUNCH_BOTH(MODEL, GRAD)                                                          \
+  hipLaunchKernelGGL((cell_kernel_both<MODEL, GRAD, VIR>), dim3(nb), dim3(kCellBlock), 0,      \
+                     stream, a.pos, a.rvecs, a.rvecs_stride, a.blk_sys, a.cnode, a.cwrap,      \
+                     a.blk_row, a.tab, a.kt, a.nstates, a.max_states, a.W, a.cbase, a.rstride, \
+                     a.gcell, a.partials, 0, nb, a.xmat, a.xmat2, a.xmat2_on)
 #define MM_LAUNCH_PAIR(MODEL, GRAD)                                                          \
   do {                                                                                       \
-    if (multi_lead) {                                                                        \
+    if (both) {                                                                              \
+      MM_LAUNCH_BOTH(MODEL, GRAD);                                                           \
+    } else if (multi_lead) {                                                                 \
       if (mn > 0) MM_LAUNCH_CELL(MODEL, GRAD, true, stream, m0, mn);                        \
       if (sn > 0) MM_LAUNCH_CELL(MODEL, GRAD, false, ms, s0, sn);                           \
     } else {                                                                                 \
@@ -520,6 +555,7 @@ hipError_t launch_cells(const ForceArgs& a, hipStream_t stream) {
   }
 #undef MM_LAUNCH_MODEL
 #undef MM_LAUNCH_PAIR
+#undef MM_LAUNCH_BOTH
 #undef MM_LAUNCH_CELL
   e = hipGetLastError();
   if (e == hipSuccess && fork) {

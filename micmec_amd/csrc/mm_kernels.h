@@ -7,7 +7,8 @@
 namespace mm {
 
 constexpr int kCellBlock = 256;
-// Cell passes of more blocks than this run their two instances concurrently.
+// Cell passes of more blocks than this run their two instances concurrently; smaller ones run
+// both kinds of block in one launch (cell_kernel_both).
 constexpr int kForkMinBlocks = 512;
 // Node gathers over at most this many 256-node blocks (at most four waves a SIMD) load all of
 // a node's records at once (sum_node_records<true>): latency bounds such a pass. NVT-NHC / NPT
